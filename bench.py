@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the implicit-geometry hot path on one MI355X.
+
+A step is one pass of the path over one synthetic grid: evaluate the
+spatially varying Schwarz-P ∩ sphere lattice at 512^3 (FP32, sign words as a
+by-product) and extract its isosurface into indexed vertex/index buffers
+(marching cubes with shared-edge dedup).  Inputs are resident in HBM; output
+buffers are preallocated from a warm-up run's counts.
+
+Prints one JSON line:
+  value       Gvoxels/s = grid points / (field + extraction) time per step
+  roofline    the dominant kernel (field kernel, VALU-bound): algorithmic flops
+              per launch (flop/point contract x points) / its average launch
+              time measured with HIP events on its stream, against the FP32
+              vector peak; the extraction's HBM figure is reported beside it
+  cpu_baseline the oracle (host restatement of the reference arithmetic, OpenMP)
+              timed on this host on a bounded z-slab of the same workload
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import gpucadforam_amd as gc  # noqa: E402
+from gpucadforam_amd import configs as CF  # noqa: E402
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+# algorithmic flops per point of the varying Schwarz-P workload (charges: FMA 2,
+# add/mul/min/max/abs/div/sqrt 1, sin or cos 16, sincos 24): coordinates 3,
+# rotation angle 1 + sincos 24, rotate 6, wavenumber 3, arguments 3, 3 cos 48,
+# sum 2, thickness 8 x (5 + 1 + 16 + 2) + 2 = 194, sheet 2, sphere 10, max 1
+FLOP_PER_POINT = {"varying_schwarz_p": 297, "plain_gyroid": 82, "bent_diamond": 144}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-slab", type=int, default=64)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    return ap.parse_args()
+
+
+class Pipeline:
+    def __init__(self, p, stream):
+        self.p = p
+        self.stream = stream
+        n = p.npoints()
+        self.field = torch.empty(n, dtype=torch.float32, device="cuda")
+        self.signs = torch.empty(gc.sign_words(p.nx, p.ny, p.nz), dtype=torch.int64,
+                                 device="cuda")
+        wsb = gc.extract_workspace_bytes(p.nx, p.ny, p.nz)
+        self.ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+        self.counts = torch.zeros(2, dtype=torch.int64, device="cuda")
+        # size outputs from one counted run
+        gc.lattice_field(p, out=self.field, signs=self.signs, stream=stream)
+        m = gc.extract(self.field, p.nx, p.ny, p.nz, signs=self.signs, workspace=self.ws,
+                       stream=stream)
+        self.nv, self.nt = m.n_vertices, m.n_triangles
+        self.verts = torch.empty((self.nv + 1024, 3), dtype=torch.float32, device="cuda")
+        self.tris = torch.empty((self.nt + 1024, 3), dtype=torch.int32, device="cuda")
+        a = gc.ExtractArgs()
+        a.field, a.signs = self.field.data_ptr(), self.signs.data_ptr()
+        a.nx, a.ny, a.nz = p.nx, p.ny, p.nz
+        a.iso = 0.0
+        a.origin = gc.Float3(0, 0, 0)
+        a.spacing = gc.Float3(1 / (p.nx - 1), 1 / (p.ny - 1), 1 / (p.nz - 1))
+        a.vertices, a.vert_capacity = self.verts.data_ptr(), self.verts.shape[0]
+        a.triangles, a.tri_capacity = self.tris.data_ptr(), self.tris.shape[0]
+        a.normals = None
+        a.workspace, a.workspace_bytes = self.ws.data_ptr(), wsb
+        a.d_counts = self.counts.data_ptr()
+        self.args = a
+        self.h = gc.stream_handle(stream)
+
+    def field_pass(self):
+        gc.check(gc.lib.gpucad_lattice_field(gc.C.byref(self.p), gc.ptr(self.field),
+                                             gc.ptr(self.signs), 0.0, self.h))
+
+    def extract_pass(self):
+        gc.check(gc.lib.gpucad_extract(gc.C.byref(self.args), None, self.h))
+
+
+def cpu_baseline(p, slab, threads, budget_s=12.0):
+    """Oracle field + indexed extraction on z-slabs of the same workload: one
+    warm-up slab, then timed slabs (median) until about budget_s of CPU work."""
+    from oracle import oracle as orc
+    nthreads = orc.threads(threads or None)
+    spacing = (1 / (p.nx - 1), 1 / (p.ny - 1), 1 / (p.nz - 1))
+    z1 = min(p.nz, slab + 1)
+    f = orc.lattice_field_f32(p, 0, z1, nthreads)
+    orc.extract_indexed(f, p.nx, p.ny, z1, spacing=spacing, nthreads=nthreads)
+    runs, spent, z0 = [], 0.0, 0
+    while spent < budget_s and len(runs) < 15:
+        z0 = (z0 + slab) % max(1, p.nz - slab - 1)
+        t0 = time.perf_counter()
+        f = orc.lattice_field_f32(p, z0, z0 + slab + 1, nthreads)
+        t1 = time.perf_counter()
+        orc.extract_indexed(f, p.nx, p.ny, slab + 1, spacing=spacing, nthreads=nthreads)
+        t2 = time.perf_counter()
+        runs.append((t2 - t0, t1 - t0, t2 - t1))
+        spent += t2 - t0
+    runs.sort()
+    tot, tf, te = runs[len(runs) // 2]
+    pts = p.nx * p.ny * (slab + 1)
+    return {
+        "value": round(pts / tot / 1e9, 5),
+        "unit": "Gvoxels/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": "median of %d z-slabs of %d planes of the %dx%dx%d workload (%d points each, "
+                  "%.1f s total): oracle field %.3f s + indexed extraction %.3f s per slab"
+                  % (len(runs), slab + 1, p.nx, p.ny, p.nz, pts, spent, tf, te),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    torch.cuda.init()
+    gc.init(0)
+    p = CF.varying_schwarz_p(args.n)
+    stream = torch.cuda.Stream()
+    pipe = Pipeline(p, stream)
+    for _ in range(args.warmup):
+        pipe.field_pass()
+        pipe.extract_pass()
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    start.record(stream)
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        pipe.field_pass()
+        ev[k][1].record(stream)
+        pipe.extract_pass()
+        ev[k][2].record(stream)
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms = start.elapsed_time(end) / args.steps
+    field_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / args.steps
+    extract_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / args.steps
+    counts = pipe.counts.cpu().tolist()
+    npts = p.npoints()
+    flops = FLOP_PER_POINT["varying_schwarz_p"] * npts
+    field_tflops = flops / (field_ms * 1e-3) / 1e12
+    ext_bytes = 4 * npts + 12 * counts[0] + 12 * counts[1]
+    ext_gbs = ext_bytes / (extract_ms * 1e-3) / 1e9
+    rec = {
+        "metric": "Gvoxels/s field+marching-cubes at 512^3",
+        "value": round(npts / (ms * 1e-3) / 1e9, 3),
+        "unit": "Gvoxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded lattice parameters, seed 0x6770)",
+        "config": {"workload": "varying_schwarz_p %dx%dx%d: period 16->64 voxels along x, "
+                               "rotation 0->90 deg along z, 8-mode Fourier thickness 0.2-0.6, "
+                               "sheet, intersect sphere r=0.45; indexed extraction"
+                               % (p.nx, p.ny, p.nz),
+                   "grid": [p.nx, p.ny, p.nz]},
+        "roofline": {"bound": "mfma" if False else "valu", "achieved": round(field_tflops, 2),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(field_tflops / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "kernel": "lattice_field_kernel",
+                     "flop_per_point": FLOP_PER_POINT["varying_schwarz_p"],
+                     "launch_ms": round(field_ms, 4)},
+        "extraction": {"ms": round(extract_ms, 4), "algorithmic_bytes": ext_bytes,
+                       "achieved_GBs": round(ext_gbs, 1),
+                       "frac_hbm": round(ext_gbs / PEAK_HBM_GBS, 4)},
+        "mtri_per_s": round(counts[1] / (extract_ms * 1e-3) / 1e6, 1),
+        "n_vertices": counts[0],
+        "n_triangles": counts[1],
+        "wall_s": round(wall, 3),
+    }
+    if not args.no_cpu:
+        rec["cpu_baseline"] = cpu_baseline(p, args.cpu_slab, args.cpu_threads, args.cpu_budget)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

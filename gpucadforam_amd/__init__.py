@@ -1,0 +1,276 @@
+"""MI355X implicit-geometry engine: Python binding of libgpucad.so.
+
+The product is the C-ABI library built from csrc/ (HIP kernels for gfx950);
+this module only loads it with ctypes and offers thin helpers that pass torch
+device tensors (device memory and streams are plumbing here).  There is no
+fallback: when libgpucad.so is missing, importing this module raises.
+
+torch is imported before the library is loaded so that the library binds to
+the HIP runtime torch already loaded (both carry the soname libamdhip64.so.7);
+one runtime per process keeps device pointers interchangeable.
+"""
+import ctypes as C
+import os
+
+import torch  # noqa: F401  (must precede loading libgpucad.so)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpucad.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        "libgpucad.so not found at %s; build it with `python -c 'import __graft_entry__ as g; "
+        "g.build()'` (hipcc --offload-arch=gfx950)" % LIB_PATH)
+
+lib = C.CDLL(LIB_PATH)
+
+# ---------------------------------------------------------------------------
+# status
+# ---------------------------------------------------------------------------
+OK, E_ARG, E_OVERFLOW, E_OOM, E_HIP, E_BUSY, E_NOTFOUND, E_IO = range(8)
+TPMS_GYROID, TPMS_SCHWARZ_P, TPMS_DIAMOND = 0, 1, 2
+PRIM_NONE, PRIM_SPHERE, PRIM_BOX = 0, 1, 2
+OP_NONE, OP_INTERSECT, OP_UNION, OP_SUBTRACT = 0, 1, 2, 3
+MAX_MODES = 8
+
+
+class GpucadError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("gpucad status %d (%s): %s" % (status, status_string(status), msg))
+        self.status = status
+
+
+class Uint3(C.Structure):
+    _fields_ = [("x", C.c_uint32), ("y", C.c_uint32), ("z", C.c_uint32)]
+
+
+class Float3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class Float4(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
+
+
+class LatticeParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("tpms", C.c_int32),
+        ("period0", C.c_float), ("period1", C.c_float),
+        ("rot0", C.c_float), ("rot1", C.c_float),
+        ("bend_radius", C.c_float),
+        ("sheet", C.c_int32),
+        ("t0", C.c_float),
+        ("n_modes", C.c_int32),
+        ("mode_k", (C.c_int32 * 3) * MAX_MODES),
+        ("mode_a", C.c_float * MAX_MODES),
+        ("mode_phi", C.c_float * MAX_MODES),
+        ("prim", C.c_int32),
+        ("op", C.c_int32),
+        ("prim_center", C.c_float * 3),
+        ("prim_radius", C.c_float),
+        ("prim_half", C.c_float * 3),
+    ]
+
+    def npoints(self):
+        return self.nx * self.ny * self.nz
+
+
+class ExtractArgs(C.Structure):
+    _fields_ = [
+        ("field", C.c_void_p),
+        ("signs", C.c_void_p),
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("iso", C.c_float),
+        ("origin", Float3),
+        ("spacing", Float3),
+        ("vertices", C.c_void_p),
+        ("vert_capacity", C.c_uint64),
+        ("triangles", C.c_void_p),
+        ("tri_capacity", C.c_uint64),
+        ("normals", C.c_void_p),
+        ("workspace", C.c_void_p),
+        ("workspace_bytes", C.c_uint64),
+        ("d_counts", C.c_void_p),
+    ]
+
+
+class MeshCounts(C.Structure):
+    _fields_ = [("n_vertices", C.c_uint64), ("n_triangles", C.c_uint64)]
+
+
+_vp, _u32, _i32, _u64, _f = C.c_void_p, C.c_uint32, C.c_int32, C.c_uint64, C.c_float
+
+# signatures of every exported entry point (argtypes, restype)
+SIGNATURES = {
+    "gpucad_version": ([], C.c_int),
+    "gpucad_status_string": ([C.c_int], C.c_char_p),
+    "gpucad_last_error": ([], C.c_char_p),
+    "gpucad_init": ([C.c_int], C.c_int),
+    "gpucad_shutdown": ([], C.c_int),
+    "gpucad_mc_tables": ([_vp, _vp, _vp], C.c_int),
+    "gpucad_sign_words": ([_i32, _i32, _i32], _u64),
+    "gpucad_lattice_field": ([C.POINTER(LatticeParams), _vp, _vp, _f, _vp], C.c_int),
+    "gpucad_lattice_field_lz": ([C.POINTER(LatticeParams), _vp, _vp, _f, _i32, _vp], C.c_int),
+    "gpucad_extract_workspace_bytes": ([_i32, _i32, _i32], _u64),
+    "gpucad_extract": ([C.POINTER(ExtractArgs), C.POINTER(MeshCounts), _vp], C.c_int),
+    "gpucad_create_lattice": ([_vp, _u32, _u32, _u32, _u32, _u32], C.c_int),
+    "gpucad_sphere_with_center": ([_vp, Float3, _f, _f, _i32, _i32, _i32, _f, _f, _f, _i32], C.c_int),
+    "gpucad_cuboid": ([_vp, Float3, Float3, _f, _f, _f, _i32, _i32, _i32, _f, _f, _f], C.c_int),
+    "gpucad_cuboid_shell": ([_vp, Float3, Float3, _f, _f, _f, _f, _i32, _i32, _i32, _f, _f, _f],
+                            C.c_int),
+    "gpucad_pyramid_frustum": ([_vp, Float3, Float3, _f, _f, _f, _f, _f, _i32, _i32, _i32, _f, _f,
+                                _f], C.c_int),
+    "gpucad_torus_with_center": ([_vp, Float3, Float3, _f, _f, _i32, _i32, _i32, _f, _f, _f],
+                                 C.c_int),
+    "gpucad_cone_with_base_radius_height": ([_vp, Float3, Float3, _f, _f, _i32, _i32, _i32, _f, _f,
+                                             _f], C.c_int),
+    "gpucad_cone_frustum": ([_vp, Float3, Float3, _f, _f, _f, _i32, _i32, _i32, _f, _f, _f],
+                            C.c_int),
+    "gpucad_distance_from_line": ([_vp, Float3, Float3, _f, _f, _f, _i32, _i32, _i32, _f, _f, _f,
+                                   _i32], C.c_int),
+    "gpucad_normalise_four": ([_vp, _vp, _vp, _u64, _i32, _i32, _i32, _f, _f], C.c_int),
+    "gpucad_primitive_field_dynamic": ([_vp, _vp, _i32, _i32, _i32], C.c_int),
+    "gpucad_classify_voxel_lattice_new": ([_vp, _vp, _vp, Uint3, Uint3, Uint3, _u32, Float3, _f],
+                                          C.c_int),
+    "gpucad_exclusive_scan_u32": ([_vp, _vp, _u32], C.c_int),
+    "gpucad_compact_voxels": ([_vp, _vp, _vp, _u32], C.c_int),
+    "gpucad_generate_triangles_lattice_new": (
+        [_vp, _vp, _vp, _vp, _vp, Uint3, Uint3, Uint3, Float3, Float3, _f, _u32, _u32, _u32, _vp,
+         _vp, _f, _f, _f, _f], C.c_int),
+    "gpucad_generate_triangles_lattice_newone": (
+        [_vp, _vp, _vp, _vp, _vp, Uint3, Uint3, Uint3, Float3, Float3, _f, _u32, _u32, _u32, _vp,
+         _f, _f], C.c_int),
+    "gpucad_generate_triangles_standard": (
+        [_vp, _vp, _vp, _vp, _vp, Uint3, Uint3, Uint3, Float3, Float3, _f, _u32, _u32], C.c_int),
+    "gpucad_compute_isosurface_lattice": (
+        [_vp, _vp, _vp, _f, _u32, _vp, _vp, _vp, _vp, Uint3, Uint3, Uint3, Float3, Float3,
+         C.POINTER(_u32), C.POINTER(_u32), _vp, _u32, _vp, _vp, _f, _f, _f, _f], C.c_int),
+    "gpucad_compute_isosurface_latticeone": (
+        [_vp, _vp, _vp, _f, _u32, _vp, _vp, _vp, _vp, Uint3, Uint3, Uint3, Float3, Float3,
+         C.POINTER(_u32), C.POINTER(_u32), _vp, _u32, _vp, _f, _f], C.c_int),
+    "gpucad_compute_isosurface_standard": (
+        [_vp, _vp, _vp, _f, _u32, _vp, _vp, _vp, _vp, Uint3, Uint3, Uint3, Float3, Float3,
+         C.POINTER(_u32), C.POINTER(_u32), _vp, _u32], C.c_int),
+    "gpucad_file_write_obj": ([_vp, _u32, C.c_char_p], C.c_int),
+    "gpucad_write_obj_indexed": ([_vp, _u64, _vp, _u64, C.c_char_p, _i32], C.c_int),
+}
+
+for _name, (_args, _res) in SIGNATURES.items():
+    _fn = getattr(lib, _name)
+    _fn.argtypes = _args
+    _fn.restype = _res
+
+
+def status_string(status):
+    return lib.gpucad_status_string(status).decode()
+
+
+def last_error():
+    return lib.gpucad_last_error().decode()
+
+
+def check(status):
+    if status != OK:
+        raise GpucadError(status, last_error())
+    return status
+
+
+# ---------------------------------------------------------------------------
+# helpers over torch tensors
+# ---------------------------------------------------------------------------
+def ptr(t):
+    """Device (or host) address of a tensor, or None."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def init(device=0):
+    check(lib.gpucad_init(device))
+
+
+def sign_words(nx, ny, nz):
+    return int(lib.gpucad_sign_words(nx, ny, nz))
+
+
+def lattice_field(params, out=None, signs=None, iso=0.0, stream=None, lz=0):
+    """Evaluates the lattice field into `out` (float32 tensor, nz*ny*nx) and,
+    if given, the sign words into `signs` (int64 tensor of sign_words())."""
+    n = params.npoints()
+    if out is None:
+        out = torch.empty(n, dtype=torch.float32, device="cuda")
+    assert out.numel() >= n and out.dtype == torch.float32 and out.is_cuda
+    if signs is not None:
+        assert signs.dtype == torch.int64 and signs.numel() >= sign_words(params.nx, params.ny,
+                                                                            params.nz)
+    check(lib.gpucad_lattice_field_lz(C.byref(params), ptr(out), ptr(signs), iso, lz,
+                                      stream_handle(stream)))
+    return out
+
+
+def extract_workspace_bytes(nx, ny, nz):
+    return int(lib.gpucad_extract_workspace_bytes(nx, ny, nz))
+
+
+class Mesh:
+    """Indexed mesh in device memory: vertices (V, 3) float32, triangles (T, 3) int32."""
+
+    def __init__(self, vertices, triangles, nv, nt):
+        self.vertices = vertices
+        self.triangles = triangles
+        self.n_vertices = nv
+        self.n_triangles = nt
+
+
+def extract(field, nx, ny, nz, iso=0.0, origin=(0.0, 0.0, 0.0), spacing=None, signs=None,
+            vert_capacity=None, tri_capacity=None, workspace=None, stream=None):
+    """Indexed marching cubes of a device field; returns a Mesh (synchronous)."""
+    if spacing is None:
+        spacing = (1.0 / (nx - 1), 1.0 / (ny - 1), 1.0 / (nz - 1))
+    if vert_capacity is None:
+        vert_capacity = max(1024, nx * ny * nz // 4)
+    if tri_capacity is None:
+        tri_capacity = 2 * vert_capacity
+    wsb = extract_workspace_bytes(nx, ny, nz)
+    if workspace is None or workspace.numel() < wsb:
+        workspace = torch.empty(wsb, dtype=torch.uint8, device=field.device)
+    for _ in range(2):
+        verts = torch.empty((vert_capacity, 3), dtype=torch.float32, device=field.device)
+        tris = torch.empty((tri_capacity, 3), dtype=torch.int32, device=field.device)
+        a = ExtractArgs()
+        a.field = field.data_ptr()
+        a.signs = None if signs is None else signs.data_ptr()
+        a.nx, a.ny, a.nz = nx, ny, nz
+        a.iso = iso
+        a.origin = Float3(*origin)
+        a.spacing = Float3(*spacing)
+        a.vertices = verts.data_ptr()
+        a.vert_capacity = vert_capacity
+        a.triangles = tris.data_ptr()
+        a.tri_capacity = tri_capacity
+        a.normals = None
+        a.workspace = workspace.data_ptr()
+        a.workspace_bytes = workspace.numel()
+        a.d_counts = None
+        counts = MeshCounts()
+        st = lib.gpucad_extract(C.byref(a), C.byref(counts), stream_handle(stream))
+        if st == E_OVERFLOW and counts.n_vertices < 2**32:
+            vert_capacity = max(vert_capacity, int(counts.n_vertices))
+            tri_capacity = max(tri_capacity, int(counts.n_triangles))
+            continue
+        check(st)
+        nv, nt = int(counts.n_vertices), int(counts.n_triangles)
+        return Mesh(verts[:nv], tris[:nt], nv, nt)
+    raise GpucadError(E_OVERFLOW, "capacity retry failed")
+
+
+def u3(x, y, z):
+    return Uint3(x, y, z)
+
+
+def f3(x, y, z):
+    return Float3(x, y, z)

@@ -1,0 +1,60 @@
+// Shared host-side plumbing of libgpucad: status reporting, HIP error checks,
+// per-device state (marching-cubes tables in device memory).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/gpucad.h"
+
+static_assert(sizeof(gc_uint3) == sizeof(uint3), "gc_uint3 must mirror uint3");
+static_assert(sizeof(gc_float3) == sizeof(float3), "gc_float3 must mirror float3");
+static_assert(sizeof(gc_float4) == sizeof(float4), "gc_float4 must mirror float4");
+
+namespace gc {
+
+// Records `msg` as the calling thread's last error and returns `status`.
+int fail(int status, const char *fmt, ...);
+void clear_error();
+
+#define GC_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t gc_err_ = (call);                                                  \
+        if (gc_err_ != hipSuccess)                                                    \
+            return ::gc::fail(GPUCAD_E_HIP, "%s failed: %s (%s:%d)", #call,          \
+                              hipGetErrorString(gc_err_), __FILE__, __LINE__);        \
+    } while (0)
+
+#define GC_CHECK_LAUNCH() GC_HIP(hipGetLastError())
+
+#define GC_ARG(cond, ...)                                                             \
+    do {                                                                              \
+        if (!(cond)) return ::gc::fail(GPUCAD_E_ARG, __VA_ARGS__);                    \
+    } while (0)
+
+// Device-resident copies of the marching-cubes tables (one set per device).
+struct DeviceTables {
+    uint64_t *tri_packed = nullptr;  // 256 x uint64: 15 edge nibbles + count in nibble 15
+    uint8_t *num_verts = nullptr;    // 256
+    uint8_t *tri_table = nullptr;    // 256*16 (255 terminated)
+};
+
+// Returns the tables of the current device, initialising them on first use.
+int device_tables(const DeviceTables **out);
+
+// Host tables (decoded once from the compact encoding in mc_tables.cpp).
+const uint8_t *host_tri_table();     // 256*16
+const uint8_t *host_num_verts();     // 256
+const uint16_t *host_edge_mask();    // 256
+const uint64_t *host_tri_packed();   // 256
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Scratch memory owned by the library for drop-in calls (grown on demand,
+// one arena per device, never shrunk).
+int scratch(size_t bytes, void **out);
+
+}  // namespace gc

@@ -1,0 +1,330 @@
+/*
+ * gpucad.h — C-ABI of the MI355X implicit-geometry engine (libgpucad.so).
+ *
+ * The library evaluates implicit lattice / primitive fields on the GPU,
+ * extracts their isosurface with marching cubes and assembles vertex and
+ * index buffers.  Every entry point takes plain pointers and sizes; device
+ * pointers are HIP device allocations owned by the caller unless stated.
+ *
+ * Two families of entry points:
+ *   - "drop-in" entry points mirror one method of the reference's C++ classes
+ *     (cited per declaration) with the same argument meaning and the same
+ *     synchronous completion (outputs are valid when the call returns).
+ *     C++ wrappers with the reference's exact class/method signatures live in
+ *     gpucad_compat.hpp.
+ *   - "native" entry points (gpucad_lattice_field, gpucad_extract, batch,
+ *     server) are asynchronous on the caller's HIP stream unless a host
+ *     output pointer asks for completion.
+ *
+ * Every function returns a status code (GPUCAD_OK on success); the message of
+ * the last failure on the calling thread is available from gpucad_last_error().
+ * Stream arguments are hipStream_t passed as void*; NULL means the null stream.
+ */
+#ifndef GPUCAD_H_
+#define GPUCAD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------ */
+enum {
+    GPUCAD_OK = 0,
+    GPUCAD_E_ARG = 1,       /* invalid argument (sizes, NULL pointers, ranges) */
+    GPUCAD_E_OVERFLOW = 2,  /* output capacity too small; counts are still reported */
+    GPUCAD_E_OOM = 3,       /* device or host allocation failed / budget exceeded */
+    GPUCAD_E_HIP = 4,       /* a HIP runtime call failed */
+    GPUCAD_E_BUSY = 5,      /* resource in use (server slots full, not ready) */
+    GPUCAD_E_NOTFOUND = 6,  /* unknown handle / ticket */
+    GPUCAD_E_IO = 7         /* file could not be written */
+};
+
+/* POD mirrors of the HIP/CUDA vector types used by the reference's
+ * signatures; layout-identical to uint3/float3/float4 (checked by
+ * static_assert in the implementation). */
+typedef struct gc_uint3 { uint32_t x, y, z; } gc_uint3;
+typedef struct gc_float3 { float x, y, z; } gc_float3;
+typedef struct gc_float4 { float x, y, z, w; } gc_float4;
+
+/* ------------------------------------------------------------------------ */
+/* Library                                                                   */
+/* ------------------------------------------------------------------------ */
+int gpucad_version(void);                  /* 10000*major + 100*minor + patch */
+const char *gpucad_status_string(int status);
+const char *gpucad_last_error(void);       /* thread-local; "" when none */
+
+/* Selects the HIP device for the calling thread and uploads the marching-cubes
+ * tables (once per device).  Replaces MarchingCubeCuda::allocateTextures_s
+ * (src/MarchingCubes_kernel.cu:24-68), which uploads triTable/numVertsTable
+ * into texture objects before any extraction. */
+int gpucad_init(int device);
+/* Releases per-device state created by gpucad_init.  Replaces
+ * MarchingCubeCuda::destroyAllTextureObjects (src/MarchingCubes_kernel.cu:71-75). */
+int gpucad_shutdown(void);
+
+/* Host copy of the 256-case tables (no GPU needed): tri_table[256*16]
+ * (edge ids, 255 = end), num_verts[256], edge_mask[256] (12-bit crossing
+ * mask).  Same data as src/tables.h:11-45, 49-307, 311-569. */
+int gpucad_mc_tables(uint8_t *tri_table, uint8_t *num_verts, uint16_t *edge_mask);
+
+/* ------------------------------------------------------------------------ */
+/* Native lattice field                                                      */
+/* ------------------------------------------------------------------------ */
+enum { GC_TPMS_GYROID = 0, GC_TPMS_SCHWARZ_P = 1, GC_TPMS_DIAMOND = 2 };
+enum { GC_PRIM_NONE = 0, GC_PRIM_SPHERE = 1, GC_PRIM_BOX = 2 };
+enum { GC_OP_NONE = 0, GC_OP_INTERSECT = 1, GC_OP_UNION = 2, GC_OP_SUBTRACT = 3 };
+#define GC_MAX_MODES 8
+
+/* A spatially varying TPMS lattice on an (nx, ny, nz)-point grid over the
+ * unit cube: point (x,y,z) sits at p = (x*hx, y*hy, z*hz), h = 1/(n-1).
+ *
+ *  phase coordinates (voxel units, revolutions after division by the period):
+ *    no bend:  (u, v) = c + R(theta(z)) * ((x, y) - c),  c = ((nx-1)/2, (ny-1)/2),
+ *              theta(z) = rot0 + (rot1 - rot0) * z / (nz-1)   (radians),  w = z
+ *    bend:     a = p_xy - C, C = (1/2, 1/2 - R_b), r = |a|, phi = atan2(a.x, a.y),
+ *              u = (C.x + R_b*phi)/hx, v = (C.y + r)/hy, w = z
+ *    P(x) = period0 + (period1 - period0) * x/(nx-1) voxels;  X = u/P, Y = v/P, Z = w/P
+ *  tpms f:   gyroid  sin X cos Y + sin Y cos Z + sin Z cos X
+ *            P       cos X + cos Y + cos Z
+ *            diamond sXsYsZ + sXcYcZ + cXsYcZ + cXcYsZ        (angles 2*pi*X, ...)
+ *  thickness t(p) = t0 + sum_m mode_a[m] cos(2*pi*(k_m . p + mode_phi[m]))
+ *  g = sheet ? |f| - t : f
+ *  primitive s: sphere |p - c| - radius;  box: exact box SDF with half extents
+ *  G = op(g, s): intersect max(g,s), union min(g,s), subtract max(g,-s), none g
+ * Inside is G < iso.  Rotation and bend are exclusive. */
+typedef struct gc_lattice_params {
+    int32_t nx, ny, nz;
+    int32_t tpms;
+    float period0, period1;
+    float rot0, rot1;
+    float bend_radius;
+    int32_t sheet;
+    float t0;
+    int32_t n_modes;
+    int32_t mode_k[GC_MAX_MODES][3];
+    float mode_a[GC_MAX_MODES];
+    float mode_phi[GC_MAX_MODES];
+    int32_t prim;
+    int32_t op;
+    float prim_center[3];
+    float prim_radius;
+    float prim_half[3];
+} gc_lattice_params;
+
+/* 64-bit words per sign row: bit b of word w of row r = y + ny*z is
+ * [G(64w+b, y, z) < iso]; padding bits are zero. */
+uint64_t gpucad_sign_words(int32_t nx, int32_t ny, int32_t nz);
+
+/* Evaluates G into d_field (nx*ny*nz floats, x fastest).  If d_signs is not
+ * NULL it also writes the sign words for isovalue iso (gpucad_sign_words()
+ * words), which lets gpucad_extract skip its own classification read. */
+int gpucad_lattice_field(const gc_lattice_params *p, float *d_field, uint64_t *d_signs,
+                         float iso, void *stream);
+/* Same, with the number of z planes each workgroup walks (1..128; 0 = default
+ * 32).  The batch autotuner sweeps it. */
+int gpucad_lattice_field_lz(const gc_lattice_params *p, float *d_field, uint64_t *d_signs,
+                            float iso, int32_t lz, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Native extraction: indexed mesh with shared-edge vertex dedup            */
+/* ------------------------------------------------------------------------ */
+/* Vertex k of the output is the k-th crossing grid edge in canonical order
+ * (edge id 3*pointIndex + axis, pointIndex = x + nx*(y + ny*z)), placed at
+ * p0 + t*(p1 - p0), t = (iso - f0)/(f1 - f0), low end first.  Triangle j is
+ * the j-th triangle in cell order (cell index x + (nx-1)*(y + (ny-1)*z)) and
+ * triTable order within the cell; its three entries are vertex indices. */
+typedef struct gc_extract_args {
+    const float *field;        /* device, nx*ny*nz */
+    const uint64_t *signs;     /* device, optional (NULL: computed into the workspace) */
+    int32_t nx, ny, nz;
+    float iso;
+    gc_float3 origin;          /* position of point (0,0,0) */
+    gc_float3 spacing;         /* point spacing per axis */
+    float *vertices;           /* device float3[vert_capacity] */
+    uint64_t vert_capacity;
+    uint32_t *triangles;       /* device uint32[3*tri_capacity] */
+    uint64_t tri_capacity;
+    float *normals;            /* device float3[vert_capacity] or NULL: per-vertex
+                                  normalised central-difference gradient */
+    void *workspace;           /* device, >= gpucad_extract_workspace_bytes() */
+    uint64_t workspace_bytes;
+    uint64_t *d_counts;        /* device uint64[2] = {vertices, triangles}, or NULL */
+} gc_extract_args;
+
+typedef struct gc_mesh_counts { uint64_t n_vertices, n_triangles; } gc_mesh_counts;
+
+uint64_t gpucad_extract_workspace_bytes(int32_t nx, int32_t ny, int32_t nz);
+/* Enqueues the extraction on stream.  If counts is not NULL the call waits for
+ * completion, fills counts and returns GPUCAD_E_OVERFLOW when a capacity was
+ * too small (nothing is written past a capacity in any case). */
+int gpucad_extract(const gc_extract_args *a, gc_mesh_counts *counts, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in field entry points                                               */
+/* ------------------------------------------------------------------------ */
+/* Fft_lattice::create_lattice (src/lattice_files/Fft_lattice.h:97,
+ * kernel src/lattice_files/Fft_lattice.cu:12-66): unit-cell lattice types 0..5. */
+int gpucad_create_lattice(float *d_latticevol, uint32_t nx, uint32_t ny, uint32_t nz,
+                          uint32_t size, uint32_t type);
+/* Modelling::sphere_with_center (src/Modelling.h:28, src/Modelling.cu:314-372) */
+int gpucad_sphere_with_center(float *data, gc_float3 center, float radius, float thickness,
+                              int32_t nx, int32_t ny, int32_t nz, float dx, float dy, float dz,
+                              int32_t shell);
+/* Modelling::cuboid (src/Modelling.h:30, src/Modelling.cu:375-431) */
+int gpucad_cuboid(float *data, gc_float3 center, gc_float3 angles, float x_width,
+                  float y_width, float z_width, int32_t nx, int32_t ny, int32_t nz,
+                  float dx, float dy, float dz);
+/* Modelling::cuboid_shell (src/Modelling.h:32, src/Modelling.cu:435-498) */
+int gpucad_cuboid_shell(float *data, gc_float3 center, gc_float3 angles, float x_width,
+                        float y_width, float z_width, float thickness, int32_t nx,
+                        int32_t ny, int32_t nz, float dx, float dy, float dz);
+/* Modelling::pyramid_frustum (src/Modelling.h:40, src/Modelling.cu:501-567) */
+int gpucad_pyramid_frustum(float *data, gc_float3 center, gc_float3 angles,
+                           float x_width_base, float x_width_top, float y_height,
+                           float z_width_base, float z_width_top, int32_t nx, int32_t ny,
+                           int32_t nz, float dx, float dy, float dz);
+/* Modelling::torus_with_center (src/Modelling.h:34, src/Modelling.cu:569-627) */
+int gpucad_torus_with_center(float *data, gc_float3 center, gc_float3 angles,
+                             float torus_radius, float circle_radius, int32_t nx,
+                             int32_t ny, int32_t nz, float dx, float dy, float dz);
+/* Modelling::cone_with_base_radius_height (src/Modelling.h:36, src/Modelling.cu:631-694) */
+int gpucad_cone_with_base_radius_height(float *data, gc_float3 center, gc_float3 angles,
+                                        float base_radius, float cone_height, int32_t nx,
+                                        int32_t ny, int32_t nz, float dx, float dy, float dz);
+/* Modelling::cone_frustum (src/Modelling.h:38, src/Modelling.cu:698-761) */
+int gpucad_cone_frustum(float *data, gc_float3 center, gc_float3 angles, float top_radius,
+                        float bottom_radius, float height, int32_t nx, int32_t ny,
+                        int32_t nz, float dx, float dy, float dz);
+/* Modelling::distance_from_line (src/Modelling.h:26, src/Modelling.cu:244-311) */
+int gpucad_distance_from_line(float *data, gc_float3 center, gc_float3 axis, float radius,
+                              float thickness_radial, float thickness_axial, int32_t nx,
+                              int32_t ny, int32_t nz, float dx, float dy, float dz,
+                              int32_t cylinder_disc);
+/* Gratings::GPU_buffer_normalise_four (src/lattice_files/Gratings.h:50,
+ * src/lattice_files/Gratings.cu:1579-1617): a = min(0, min f), b = max(0, max f),
+ * k = (f-a)/(b-a); boundary points -> indicator 0, k 0; else indicator
+ * (iso1 <= k <= iso2).  normalised may alias in. */
+int gpucad_normalise_four(const float *in, float *indicator, float *normalised, uint64_t size,
+                          int32_t nx, int32_t ny, int32_t nz, float iso1, float iso2);
+/* Gratings::primitive_field (src/lattice_files/Gratings.h:77,
+ * src/lattice_files/Gratings.cu:1695-1734), dynamic-primitive form: sets
+ * isosurf[i] = FLT_MAX where primitive_active[i] >= 0. */
+int gpucad_primitive_field_dynamic(const float *primitive_active, float *isosurf, int32_t nx,
+                                   int32_t ny, int32_t nz);
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in extraction entry points (non-indexed float4 output)              */
+/* ------------------------------------------------------------------------ */
+/* MarchingCubeCuda::classifyVoxel_lattice_new (src/MarchingCubes_kernel.h:75,
+ * src/MarchingCubes_kernel.cu:3207-3265): per cell cube = sum [f_k < iso] 2^k,
+ * voxelVerts = numVerts[cube], voxelOccupied = voxelVerts > 0. */
+int gpucad_classify_voxel_lattice_new(uint32_t *voxel_verts, uint32_t *voxel_occupied,
+                                      const float *volume, gc_uint3 grid_size,
+                                      gc_uint3 grid_size_shift, gc_uint3 grid_size_mask,
+                                      uint32_t num_voxels, gc_float3 voxel_size, float iso);
+/* MarchingCubeCuda::ThrustScanWrapper_lattice (src/MarchingCubes_kernel.h:123,
+ * src/MarchingCubes_kernel.cu:3198-3203): exclusive uint32 scan (wrapping). */
+int gpucad_exclusive_scan_u32(uint32_t *output, const uint32_t *input, uint32_t n);
+/* MarchingCubeCuda::compactVoxels_lattice (src/MarchingCubes_kernel.h:79,
+ * src/MarchingCubes_kernel.cu:1594-1615). */
+int gpucad_compact_voxels(uint32_t *compacted, const uint32_t *voxel_occupied,
+                          const uint32_t *voxel_occupied_scan, uint32_t num_voxels);
+/* MarchingCubeCuda::generateTriangles_lattice_new (src/MarchingCubes_kernel.h:108,
+ * src/MarchingCubes_kernel.cu:3424-3588): band interpolation over two fields. */
+int gpucad_generate_triangles_lattice_new(gc_float4 *pos, gc_float4 *norm,
+                                          const uint32_t *compacted,
+                                          const uint32_t *num_verts_scanned,
+                                          const float *volume, gc_uint3 grid_size,
+                                          gc_uint3 grid_size_shift, gc_uint3 grid_size_mask,
+                                          gc_float3 voxel_size, gc_float3 grid_center,
+                                          float iso, uint32_t active_voxels,
+                                          uint32_t max_verts, uint32_t total_verts,
+                                          const float *volume_one, const float *volume_two,
+                                          float isovalue1, float isovalue2, float iso1,
+                                          float iso2);
+/* MarchingCubeCuda::generateTriangles_lattice_newone (src/MarchingCubes_kernel.h:113,
+ * src/MarchingCubes_kernel.cu:3678-3836): band interpolation over one field. */
+int gpucad_generate_triangles_lattice_newone(gc_float4 *pos, gc_float4 *norm,
+                                             const uint32_t *compacted,
+                                             const uint32_t *num_verts_scanned,
+                                             const float *volume, gc_uint3 grid_size,
+                                             gc_uint3 grid_size_shift,
+                                             gc_uint3 grid_size_mask, gc_float3 voxel_size,
+                                             gc_float3 grid_center, float iso,
+                                             uint32_t active_voxels, uint32_t max_verts,
+                                             uint32_t total_verts, const float *volume_one,
+                                             float isovalue1, float isovalue2);
+/* Textbook generate (vertexInterp4, src/MarchingCubes_kernel.cu:1619-1629,
+ * emission as in 3041-3170) over the classification of
+ * gpucad_classify_voxel_lattice_new: interpolates `volume` itself at iso. */
+int gpucad_generate_triangles_standard(gc_float4 *pos, gc_float4 *norm,
+                                       const uint32_t *compacted,
+                                       const uint32_t *num_verts_scanned,
+                                       const float *volume, gc_uint3 grid_size,
+                                       gc_uint3 grid_size_shift, gc_uint3 grid_size_mask,
+                                       gc_float3 voxel_size, gc_float3 grid_center, float iso,
+                                       uint32_t active_voxels, uint32_t max_verts);
+/* Isosurface::computeIsosurface_lattice (src/Isosurface.h:60-64,
+ * src/Isosurface.cu:401-486): classify -> scan -> compact -> scan -> generate
+ * (band, two fields).  active_voxels and total_verts are host outputs; writes
+ * past max_verts-3 are dropped while *total_verts stays unclamped. */
+int gpucad_compute_isosurface_lattice(const float *vol, gc_float4 *pos, gc_float4 *norm,
+                                      float iso, uint32_t num_voxels, uint32_t *d_voxel_verts,
+                                      uint32_t *d_voxel_verts_scan,
+                                      uint32_t *d_voxel_occupied,
+                                      uint32_t *d_voxel_occupied_scan, gc_uint3 grid_size,
+                                      gc_uint3 grid_size_shift, gc_uint3 grid_size_mask,
+                                      gc_float3 voxel_size, gc_float3 grid_center,
+                                      uint32_t *active_voxels, uint32_t *total_verts,
+                                      uint32_t *d_comp_voxel_array, uint32_t max_verts,
+                                      const float *vol_one, const float *vol_two,
+                                      float isovalue1, float isovalue2, float iso1, float iso2);
+/* Isosurface::computeIsosurface_latticeone (src/Isosurface.h:66-70,
+ * src/Isosurface.cu:488-572) */
+int gpucad_compute_isosurface_latticeone(const float *vol, gc_float4 *pos, gc_float4 *norm,
+                                         float iso, uint32_t num_voxels,
+                                         uint32_t *d_voxel_verts, uint32_t *d_voxel_verts_scan,
+                                         uint32_t *d_voxel_occupied,
+                                         uint32_t *d_voxel_occupied_scan, gc_uint3 grid_size,
+                                         gc_uint3 grid_size_shift, gc_uint3 grid_size_mask,
+                                         gc_float3 voxel_size, gc_float3 grid_center,
+                                         uint32_t *active_voxels, uint32_t *total_verts,
+                                         uint32_t *d_comp_voxel_array, uint32_t max_verts,
+                                         const float *vol_one, float isovalue1,
+                                         float isovalue2);
+/* Same orchestration with the textbook generate (gpucad_generate_triangles_standard). */
+int gpucad_compute_isosurface_standard(const float *vol, gc_float4 *pos, gc_float4 *norm,
+                                       float iso, uint32_t num_voxels, uint32_t *d_voxel_verts,
+                                       uint32_t *d_voxel_verts_scan,
+                                       uint32_t *d_voxel_occupied,
+                                       uint32_t *d_voxel_occupied_scan, gc_uint3 grid_size,
+                                       gc_uint3 grid_size_shift, gc_uint3 grid_size_mask,
+                                       gc_float3 voxel_size, gc_float3 grid_center,
+                                       uint32_t *active_voxels, uint32_t *total_verts,
+                                       uint32_t *d_comp_voxel_array, uint32_t max_verts);
+
+/* ------------------------------------------------------------------------ */
+/* .obj output                                                               */
+/* ------------------------------------------------------------------------ */
+/* File_output::file_write_obj (src/File_output.h:189, src/File_output.cu:5-81):
+ * byte-identical output — weld key int(v*1000)*0.001 per component, vertices
+ * numbered in first-seen order, faces with 3 distinct ids kept once per
+ * ordered triple, written "f a c b". */
+int gpucad_file_write_obj(const gc_float4 *d_pos, uint32_t total_verts, const char *filename);
+/* Native writer from indexed device buffers (vertices float3, triangles
+ * uint32 x3): "v x y z" lines then "f a b c" (1-based; flip_winding writes
+ * "f a c b" like the reference). */
+int gpucad_write_obj_indexed(const float *d_vertices, uint64_t n_vertices,
+                             const uint32_t *d_triangles, uint64_t n_triangles,
+                             const char *filename, int32_t flip_winding);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUCAD_H_ */

@@ -1,0 +1,216 @@
+"""ORACLE — test infrastructure only (checker and CPU baseline).
+
+ctypes binding of liboracle.so (oracle.cpp, the host restatement of the
+reference's hot-path arithmetic).  Only tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg import this module, and only to check or to time
+the CPU path; the product path never calls it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+    return LIB_PATH
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("liboracle.so not built; run make -C oracle")
+    lib = C.CDLL(LIB_PATH)
+    f32p = np.ctypeslib.ndpointer(np.float32, flags="C")
+    f64p = np.ctypeslib.ndpointer(np.float64, flags="C")
+    u32p = np.ctypeslib.ndpointer(np.uint32, flags="C")
+    u64p = np.ctypeslib.ndpointer(np.uint64, flags="C")
+    u8p = np.ctypeslib.ndpointer(np.uint8, flags="C")
+    vp = C.c_void_p
+    sig = {
+        "orc_cos_rev": ([f32p, f32p, C.c_int64], None),
+        "orc_sin_rev": ([f32p, f32p, C.c_int64], None),
+        "orc_atan2": ([f32p, f32p, f32p, C.c_int64], None),
+        "orc_tables": ([u8p, u8p], None),
+        "orc_lattice_field_f32": ([vp, f32p, C.c_int, C.c_int, C.c_int], None),
+        "orc_lattice_field_f64": ([vp, f64p, C.c_int, C.c_int, C.c_int], None),
+        "orc_signs": ([f32p, C.c_int, C.c_int, C.c_int, C.c_float, u64p], None),
+        "orc_extract_indexed": ([f32p, C.c_int, C.c_int, C.c_int, C.c_float, f32p, f32p, vp,
+                                 C.c_uint64, vp, C.c_uint64, u64p, vp, C.c_int], C.c_int),
+        "orc_classify": ([f32p, C.c_int, C.c_int, C.c_int, C.c_float, vp, vp, vp], None),
+        "orc_exclusive_scan_u32": ([u32p, u32p, C.c_uint64], None),
+        "orc_compact": ([u32p, u32p, C.c_uint64, u32p], None),
+        "orc_generate": ([C.c_int, f32p, vp, vp, C.c_int, C.c_int, C.c_int, f32p, f32p, C.c_float,
+                          C.c_float, C.c_float, C.c_float, C.c_float, u32p, u32p, C.c_uint32,
+                          C.c_uint32, f32p, f32p], None),
+        "orc_compute_isosurface": ([C.c_int, f32p, vp, vp, C.c_int, C.c_int, C.c_int, f32p, f32p,
+                                    C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                    C.c_uint32, u32p, u32p, u32p, u32p, u32p, f32p, f32p,
+                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)], C.c_int),
+        "orc_normalise_four": ([f32p, f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                C.c_float], None),
+        "orc_create_lattice": ([f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int], None),
+        "orc_primitive": ([C.c_int, f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                           C.c_float, f32p, f32p, f32p, C.c_int, C.c_int], None),
+        "orc_file_write_obj": ([f32p, C.c_uint32, C.c_char_p], C.c_int),
+        "orc_max_threads": ([], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+lib = _load()
+
+
+def threads(n=None):
+    return n if n else lib.orc_max_threads()
+
+
+def cos_rev(r):
+    r = np.ascontiguousarray(r, np.float32)
+    out = np.empty_like(r)
+    lib.orc_cos_rev(r, out, r.size)
+    return out
+
+
+def sin_rev(r):
+    r = np.ascontiguousarray(r, np.float32)
+    out = np.empty_like(r)
+    lib.orc_sin_rev(r, out, r.size)
+    return out
+
+
+def atan2(y, x):
+    y = np.ascontiguousarray(y, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(y)
+    lib.orc_atan2(y, x, out, y.size)
+    return out
+
+
+def tables():
+    tri = np.empty(256 * 16, np.uint8)
+    nv = np.empty(256, np.uint8)
+    lib.orc_tables(tri, nv)
+    return tri.reshape(256, 16), nv
+
+
+def lattice_field_f32(p, z0=0, z1=None, nthreads=None):
+    z1 = p.nz if z1 is None else z1
+    out = np.empty((z1 - z0) * p.ny * p.nx, np.float32)
+    lib.orc_lattice_field_f32(C.addressof(p), out, z0, z1, threads(nthreads))
+    return out
+
+
+def lattice_field_f64(p, z0=0, z1=None, nthreads=None):
+    z1 = p.nz if z1 is None else z1
+    out = np.empty((z1 - z0) * p.ny * p.nx, np.float64)
+    lib.orc_lattice_field_f64(C.addressof(p), out, z0, z1, threads(nthreads))
+    return out
+
+
+def signs(field, nx, ny, nz, iso=0.0):
+    W = (nx + 63) // 64
+    out = np.empty(W * ny * nz, np.uint64)
+    lib.orc_signs(np.ascontiguousarray(field, np.float32), nx, ny, nz, iso, out)
+    return out
+
+
+def extract_indexed(field, nx, ny, nz, iso=0.0, origin=(0, 0, 0), spacing=None, vcap=None,
+                    tcap=None, edge_ids=False, nthreads=None):
+    """Canonical indexed mesh: (vertices (V,3) f32, triangles (T,3) u32, edge ids or None)."""
+    field = np.ascontiguousarray(field, np.float32)
+    if spacing is None:
+        spacing = (1.0 / (nx - 1), 1.0 / (ny - 1), 1.0 / (nz - 1))
+    counts = np.zeros(2, np.uint64)
+    o = np.asarray(origin, np.float32)
+    s = np.asarray(spacing, np.float32)
+    if vcap is None:
+        lib.orc_extract_indexed(field, nx, ny, nz, iso, o, s, None, 0, None, 0, counts, None,
+                                threads(nthreads))
+        vcap, tcap = int(counts[0]), int(counts[1])
+    verts = np.zeros((max(vcap, 1), 3), np.float32)
+    tris = np.zeros((max(tcap, 1), 3), np.uint32)
+    eids = np.zeros(max(vcap, 1), np.uint64) if edge_ids else None
+    lib.orc_extract_indexed(field, nx, ny, nz, iso, o, s, verts.ctypes.data, vcap,
+                            tris.ctypes.data, tcap, counts,
+                            None if eids is None else eids.ctypes.data, threads(nthreads))
+    nv, nt = int(counts[0]), int(counts[1])
+    return verts[:nv], tris[:nt], (eids[:nv] if edge_ids else None)
+
+
+def classify(vol, nx, ny, nz, iso):
+    n = (nx - 1) * (ny - 1) * (nz - 1)
+    cube = np.empty(n, np.uint8)
+    verts = np.empty(n, np.uint32)
+    occ = np.empty(n, np.uint32)
+    lib.orc_classify(np.ascontiguousarray(vol, np.float32), nx, ny, nz, iso, cube.ctypes.data,
+                     verts.ctypes.data, occ.ctypes.data)
+    return cube, verts, occ
+
+
+def exclusive_scan_u32(a):
+    a = np.ascontiguousarray(a, np.uint32)
+    out = np.empty_like(a)
+    lib.orc_exclusive_scan_u32(a, out, a.size)
+    return out
+
+
+MODE_BAND2, MODE_BAND1, MODE_STD = 0, 1, 2
+
+
+def compute_isosurface(mode, vol, nx, ny, nz, iso, voxel_size=(1, 1, 1), center=(0, 0, 0),
+                       vol1=None, vol2=None, lo1=0.0, hi1=0.0, lo2=0.0, hi2=0.0, max_verts=None):
+    n = (nx - 1) * (ny - 1) * (nz - 1)
+    if max_verts is None:
+        max_verts = max(nx * ny * nz * 4, 300000)
+    out = {k: np.zeros(n, np.uint32) for k in ("verts", "verts_scan", "occ", "occ_scan", "comp")}
+    pos = np.zeros(4 * max_verts, np.float32)
+    norm = np.zeros(4 * max_verts, np.float32)
+    active, total = C.c_uint32(0), C.c_uint32(0)
+    keep = [np.ascontiguousarray(v, np.float32) if v is not None else None for v in (vol1, vol2)]
+    lib.orc_compute_isosurface(
+        mode, np.ascontiguousarray(vol, np.float32),
+        None if keep[0] is None else keep[0].ctypes.data,
+        None if keep[1] is None else keep[1].ctypes.data, nx, ny, nz,
+        np.asarray(voxel_size, np.float32), np.asarray(center, np.float32), iso, lo1, hi1, lo2,
+        hi2, max_verts, out["verts"], out["verts_scan"], out["occ"], out["occ_scan"], out["comp"],
+        pos, norm, C.byref(active), C.byref(total))
+    out["active"] = active.value
+    out["total"] = total.value
+    out["pos"] = pos.reshape(-1, 4)
+    out["norm"] = norm.reshape(-1, 4)
+    return out
+
+
+def normalise_four(vol, nx, ny, nz, iso1, iso2):
+    vol = np.ascontiguousarray(vol, np.float32)
+    ind = np.empty_like(vol)
+    nrm = np.empty_like(vol)
+    lib.orc_normalise_four(vol, ind, nrm, nx, ny, nz, iso1, iso2)
+    return ind, nrm
+
+
+def create_lattice(nx, ny, nz, type_, f64=False):
+    out = np.empty(nx * ny * nz, np.float32)
+    lib.orc_create_lattice(out, nx, ny, nz, type_, 1 if f64 else 0)
+    return out
+
+
+def primitive(kind, nx, ny, nz, d, center, angles, prm, flag=0, nthreads=None):
+    out = np.empty(nx * ny * nz, np.float32)
+    prm = np.asarray(list(prm) + [0.0] * (8 - len(prm)), np.float32)
+    lib.orc_primitive(kind, out, nx, ny, nz, d[0], d[1], d[2], np.asarray(center, np.float32),
+                      np.asarray(angles, np.float32), prm, flag, threads(nthreads))
+    return out
+
+
+def file_write_obj(pos4, total, filename):
+    pos4 = np.ascontiguousarray(pos4, np.float32).reshape(-1)
+    return lib.orc_file_write_obj(pos4, total, filename.encode())
