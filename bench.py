@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-slab", type=int, default=64)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--calib", action="store_true",
+                    help="also run a 512 MiB device copy (counter calibration)")
     return ap.parse_args()
 
 
@@ -155,6 +157,11 @@ def main():
     end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
+    if args.calib:
+        src = torch.empty(128 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     ms = start.elapsed_time(end) / args.steps
     field_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / args.steps
     extract_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / args.steps

@@ -104,6 +104,7 @@ _vp, _u32, _i32, _u64, _f = C.c_void_p, C.c_uint32, C.c_int32, C.c_uint64, C.c_f
 # signatures of every exported entry point (argtypes, restype)
 SIGNATURES = {
     "gpucad_version": ([], C.c_int),
+    "gpucad_abi_sizes": ([C.POINTER(C.c_uint32), C.c_int32], C.c_int),
     "gpucad_status_string": ([C.c_int], C.c_char_p),
     "gpucad_last_error": ([], C.c_char_p),
     "gpucad_init": ([C.c_int], C.c_int),

@@ -18,6 +18,7 @@
 //    Gratings::GPU_buffer_normalise_four (src/lattice_files/Gratings.cu:1089-1134,
 //    1394-1495, 1579-1617) and Gratings::primitive_field
 //    (src/lattice_files/Gratings.cu:1695-1734).
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -172,16 +173,20 @@ lattice_field_kernel(FieldConsts c, float *__restrict__ field, uint64_t *__restr
     float T0 = c.t0;
     Cplx S1 = {0.f, 0.f}, S2 = {0.f, 0.f}, S3 = {0.f, 0.f};
     if (THICK) {
-        Cplx ex[4], ey[4];
-        ex[0] = {1.f, 0.f};
-        ey[0] = {1.f, 0.f};
-        for (int k = 1; k <= 3; ++k) {
-            ex[k] = expi_rev((float)k * px);
-            ey[k] = expi_rev((float)k * py);
-        }
+        // e^{2 pi i k px} and e^{2 pi i k py} for k = 0..3, selected without
+        // dynamic register indexing (which would spill to scratch)
+        const Cplx ex1 = expi_rev(px), ex2 = expi_rev(2.0f * px), ex3 = expi_rev(3.0f * px);
+        const Cplx ey1 = expi_rev(py), ey2 = expi_rev(2.0f * py), ey3 = expi_rev(3.0f * py);
+        auto pick = [](int k, Cplx e1, Cplx e2, Cplx e3) {
+            Cplx r = {1.f, 0.f};
+            r = k == 1 ? e1 : r;
+            r = k == 2 ? e2 : r;
+            r = k == 3 ? e3 : r;
+            return r;
+        };
         Cplx S0 = {0.f, 0.f};
         for (int m = 0; m < c.n_modes; ++m) {
-            Cplx w = cmul(ex[c.mode_k[m][0]], ey[c.mode_k[m][1]]);
+            Cplx w = cmul(pick(c.mode_k[m][0], ex1, ex2, ex3), pick(c.mode_k[m][1], ey1, ey2, ey3));
             Cplx ph = {c.phi_re[m], c.phi_im[m]};
             Cplx a = cmul(w, ph);
             int kz = c.mode_k[m][2];
@@ -327,7 +332,9 @@ int launch_lattice_field(const gc_lattice_params *p, float *d_field, uint64_t *d
     int st = validate_lattice(p);
     if (st) return st;
     GC_ARG(d_field, "field pointer is NULL");
-    if (lz <= 0) lz = 32;
+    // default z run: long enough to amortise the per-column work, short enough
+    // to leave about 8 workgroups per z column of the grid
+    if (lz <= 0) lz = std::max(16, std::min(kMaxLZ, p->nz / 8));
     if (lz > kMaxLZ) lz = kMaxLZ;
     FieldConsts c;
     std::memset(&c, 0, sizeof(c));

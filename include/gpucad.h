@@ -55,6 +55,10 @@ typedef struct gc_float4 { float x, y, z, w; } gc_float4;
 /* Library                                                                   */
 /* ------------------------------------------------------------------------ */
 int gpucad_version(void);                  /* 10000*major + 100*minor + patch */
+/* sizeof of the ABI structs, for binding checks: sizes[0..n) receives
+ * gc_lattice_params, gc_extract_args, gc_mesh_counts, gc_uint3, gc_float3,
+ * gc_float4 (in that order); returns the number written. */
+int gpucad_abi_sizes(uint32_t *sizes, int32_t n);
 const char *gpucad_status_string(int status);
 const char *gpucad_last_error(void);       /* thread-local; "" when none */
 
