@@ -132,6 +132,7 @@ SIGNATURES = {
                                    _i32], C.c_int),
     "gpucad_normalise_four": ([_vp, _vp, _vp, _u64, _i32, _i32, _i32, _f, _f], C.c_int),
     "gpucad_primitive_field_dynamic": ([_vp, _vp, _i32, _i32, _i32], C.c_int),
+    "gpucad_primitive_field_fixed": ([_vp, _vp, _i32, _i32, _i32], C.c_int),
     "gpucad_classify_voxel_lattice_new": ([_vp, _vp, _vp, Uint3, Uint3, Uint3, _u32, Float3, _f],
                                           C.c_int),
     "gpucad_exclusive_scan_u32": ([_vp, _vp, _u32], C.c_int),

@@ -678,6 +678,12 @@ __global__ void primitive_dynamic_kernel(const float *active, float *iso, uint64
     if (i < n && active[i] >= 0.f) iso[i] = FLT_MAX;
 }
 
+// grid_points {int val; float t_x, t_y, t_z;}: only val is read
+__global__ void primitive_fixed_kernel(const int4 *fixed, float *iso, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && fixed[i].x > -1) iso[i] = FLT_MAX;
+}
+
 int point_launch(int nx, int ny, int nz, dim3 *grid) {
     GC_ARG(nx >= 1 && ny >= 1 && nz >= 1, "grid sizes must be positive");
     uint64_t n = (uint64_t)nx * ny * nz;
@@ -864,6 +870,20 @@ int gpucad_primitive_field_dynamic(const float *primitive_active, float *isosurf
     if (st) return st;
     hipLaunchKernelGGL(primitive_dynamic_kernel, grid, dim3(256), 0, 0, primitive_active, isosurf,
                        (uint64_t)nx * ny * nz);
+    GC_CHECK_LAUNCH();
+    GC_HIP(hipStreamSynchronize(0));
+    return GPUCAD_OK;
+}
+
+int gpucad_primitive_field_fixed(const void *primitive_fixed, float *isosurf, int32_t nx,
+                                 int32_t ny, int32_t nz) {
+    clear_error();
+    GC_ARG(primitive_fixed && isosurf, "NULL buffer");
+    dim3 grid;
+    int st = point_launch(nx, ny, nz, &grid);
+    if (st) return st;
+    hipLaunchKernelGGL(primitive_fixed_kernel, grid, dim3(256), 0, 0,
+                       (const int4 *)primitive_fixed, isosurf, (uint64_t)nx * ny * nz);
     GC_CHECK_LAUNCH();
     GC_HIP(hipStreamSynchronize(0));
     return GPUCAD_OK;

@@ -220,6 +220,12 @@ int gpucad_normalise_four(const float *in, float *indicator, float *normalised, 
  * isosurf[i] = FLT_MAX where primitive_active[i] >= 0. */
 int gpucad_primitive_field_dynamic(const float *primitive_active, float *isosurf, int32_t nx,
                                    int32_t ny, int32_t nz);
+/* Gratings::primitive_field, fixed-primitive form: sets isosurf[i] = FLT_MAX
+ * where primitive_fixed[i].val > -1.  primitive_fixed points at the
+ * reference's grid_points records {int val; float t_x, t_y, t_z;} (16 bytes,
+ * src/MarchingCubes_kernel.h:12-18). */
+int gpucad_primitive_field_fixed(const void *primitive_fixed, float *isosurf, int32_t nx,
+                                 int32_t ny, int32_t nz);
 
 /* ------------------------------------------------------------------------ */
 /* Drop-in extraction entry points (non-indexed float4 output)              */

@@ -1,0 +1,317 @@
+// gpucad_compat.hpp — the reference's C++ operator API on top of libgpucad.
+//
+// Header-only C++ classes with the method signatures of the reference's
+// hot-path classes, so that the reference's caller (class Multitopo,
+// src/main.cu) compiles against them unchanged and runs the MI355X kernels:
+//
+//   MarchingCubeCuda   src/MarchingCubes_kernel.h:34-124 (path methods)
+//   Isosurface         src/Isosurface.h:10-80 (lattice extraction methods)
+//   Modelling          src/Modelling.h:10-48 (primitive fields)
+//   Fft_lattice        src/lattice_files/Fft_lattice.h:10-24 (create_lattice)
+//   Gratings           src/lattice_files/Gratings.h:10-80 (normalise, masks)
+//   File_output        src/File_output.h:35-44 (file_write_obj)
+//
+// Semantics follow the reference: every call is synchronous; launch-shape
+// arguments (dim3 grid, threads) are accepted and ignored (the library picks
+// its own shapes); a failing call prints the library's message and exits the
+// process, as the reference's checkCudaErrors does
+// (src/commons/helper_cuda.h:583-611).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+
+#include "gpucad.h"
+
+typedef unsigned int uint;
+
+// src/MarchingCubes_kernel.h:12-18
+struct grid_points {
+    int val = 0;
+    float t_x = 0.0;
+    float t_y = 0.0;
+    float t_z = 0.0;
+};
+static_assert(sizeof(grid_points) == 16, "grid_points is a 16-byte record");
+
+namespace gpucad_compat {
+
+inline void check(int status, const char *what) {
+    if (status != GPUCAD_OK) {
+        std::fprintf(stderr, "gpucad error in %s: %s (%s)\n", what,
+                     gpucad_status_string(status), gpucad_last_error());
+        std::exit(EXIT_FAILURE);
+    }
+}
+
+inline gc_uint3 u3(uint3 v) { return gc_uint3{v.x, v.y, v.z}; }
+inline gc_float3 f3(float3 v) { return gc_float3{v.x, v.y, v.z}; }
+
+inline void ensure_init() {
+    static bool done = false;
+    if (!done) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+        check(gpucad_init(dev), "gpucad_init");
+        done = true;
+    }
+}
+
+}  // namespace gpucad_compat
+
+class MarchingCubeCuda {
+public:
+    // Uploads the case tables (uint, 255-terminated rows, as src/tables.h)
+    // and returns their device copies; also initialises the library.
+    void allocateTextures_s(uint **d_triTable, uint **d_numVertsTable) {
+        using namespace gpucad_compat;
+        ensure_init();
+        uint8_t tri[256 * 16], nv[256];
+        check(gpucad_mc_tables(tri, nv, nullptr), "gpucad_mc_tables");
+        uint tri32[256 * 16], nv32[256];
+        for (int i = 0; i < 256 * 16; ++i) tri32[i] = tri[i];
+        for (int i = 0; i < 256; ++i) nv32[i] = nv[i];
+        if (hipMalloc((void **)d_triTable, sizeof(tri32)) != hipSuccess ||
+            hipMalloc((void **)d_numVertsTable, sizeof(nv32)) != hipSuccess ||
+            hipMemcpy(*d_triTable, tri32, sizeof(tri32), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(*d_numVertsTable, nv32, sizeof(nv32), hipMemcpyHostToDevice) != hipSuccess)
+            check(GPUCAD_E_HIP, "allocateTextures_s");
+    }
+
+    void destroyAllTextureObjects() { gpucad_compat::check(gpucad_shutdown(), "gpucad_shutdown"); }
+
+    void ThrustScanWrapper_lattice(unsigned int *output, unsigned int *input,
+                                   unsigned int numElements) {
+        gpucad_compat::check(gpucad_exclusive_scan_u32(output, input, numElements),
+                             "ThrustScanWrapper_lattice");
+    }
+
+    void classifyVoxel_lattice_new(dim3, dim3, uint *voxelVerts, uint *voxelOccupied,
+                                   float *volume, uint3 gridSize, uint3 gridSizeShift,
+                                   uint3 gridSizeMask, uint numVoxels, float3 voxelSize,
+                                   float isoValue) {
+        using namespace gpucad_compat;
+        ensure_init();
+        check(gpucad_classify_voxel_lattice_new(voxelVerts, voxelOccupied, volume, u3(gridSize),
+                                                u3(gridSizeShift), u3(gridSizeMask), numVoxels,
+                                                f3(voxelSize), isoValue),
+              "classifyVoxel_lattice_new");
+    }
+
+    void compactVoxels_lattice(dim3, dim3, uint *compactedVoxelArray, uint *voxelOccupied,
+                               uint *voxelOccupiedScan, uint numVoxels) {
+        gpucad_compat::check(gpucad_compact_voxels(compactedVoxelArray, voxelOccupied,
+                                                   voxelOccupiedScan, numVoxels),
+                             "compactVoxels_lattice");
+    }
+
+    void generateTriangles_lattice_new(dim3, dim3, float4 *pos, float4 *norm,
+                                       uint *compactedVoxelArray, uint *numVertsScanned,
+                                       float *volume, uint3 gridSize, uint3 gridSizeShift,
+                                       uint3 gridSizeMask, float3 voxelSize, float3 gridcenter,
+                                       float isoValue, uint activeVoxels, uint maxVerts,
+                                       uint totalverts, float *volume_one, float *volume_two,
+                                       float isovalue1, float isovalue2, float iso1,
+                                       float iso2) {
+        using namespace gpucad_compat;
+        ensure_init();
+        check(gpucad_generate_triangles_lattice_new(
+                  (gc_float4 *)pos, (gc_float4 *)norm, compactedVoxelArray, numVertsScanned,
+                  volume, u3(gridSize), u3(gridSizeShift), u3(gridSizeMask), f3(voxelSize),
+                  f3(gridcenter), isoValue, activeVoxels, maxVerts, totalverts, volume_one,
+                  volume_two, isovalue1, isovalue2, iso1, iso2),
+              "generateTriangles_lattice_new");
+    }
+
+    void generateTriangles_lattice_newone(dim3, dim3, float4 *pos, float4 *norm,
+                                          uint *compactedVoxelArray, uint *numVertsScanned,
+                                          float *volume, uint3 gridSize, uint3 gridSizeShift,
+                                          uint3 gridSizeMask, float3 voxelSize,
+                                          float3 gridcenter, float isoValue, uint activeVoxels,
+                                          uint maxVerts, uint totalverts, float *volume_one,
+                                          float isovalue1, float isovalue2) {
+        using namespace gpucad_compat;
+        ensure_init();
+        check(gpucad_generate_triangles_lattice_newone(
+                  (gc_float4 *)pos, (gc_float4 *)norm, compactedVoxelArray, numVertsScanned,
+                  volume, u3(gridSize), u3(gridSizeShift), u3(gridSizeMask), f3(voxelSize),
+                  f3(gridcenter), isoValue, activeVoxels, maxVerts, totalverts, volume_one,
+                  isovalue1, isovalue2),
+              "generateTriangles_lattice_newone");
+    }
+};
+
+class Isosurface : public MarchingCubeCuda {
+public:
+    void computeIsosurface_lattice(float *vol, float4 *pos, float4 *norm, float &isoValue,
+                                   uint numVoxels, uint *d_voxelVerts, uint *d_voxelVertsScan,
+                                   uint *d_voxelOccupied, uint *d_voxelOccupiedScan,
+                                   uint3 gridSize, uint3 gridSizeShift, uint3 gridSizeMask,
+                                   float3 voxelSize, float3 gridcenter, uint *activeVoxels,
+                                   uint *totalVerts, uint *d_compVoxelArray, uint maxVerts,
+                                   float *vol_one, float *vol_two, float isovalue1,
+                                   float isovalue2, float iso1, float iso2) {
+        using namespace gpucad_compat;
+        ensure_init();
+        check(gpucad_compute_isosurface_lattice(
+                  vol, (gc_float4 *)pos, (gc_float4 *)norm, isoValue, numVoxels, d_voxelVerts,
+                  d_voxelVertsScan, d_voxelOccupied, d_voxelOccupiedScan, u3(gridSize),
+                  u3(gridSizeShift), u3(gridSizeMask), f3(voxelSize), f3(gridcenter),
+                  activeVoxels, totalVerts, d_compVoxelArray, maxVerts, vol_one, vol_two,
+                  isovalue1, isovalue2, iso1, iso2),
+              "computeIsosurface_lattice");
+    }
+
+    void computeIsosurface_latticeone(float *vol, float4 *pos, float4 *norm, float &isoValue,
+                                      uint numVoxels, uint *d_voxelVerts, uint *d_voxelVertsScan,
+                                      uint *d_voxelOccupied, uint *d_voxelOccupiedScan,
+                                      uint3 gridSize, uint3 gridSizeShift, uint3 gridSizeMask,
+                                      float3 voxelSize, float3 gridcenter, uint *activeVoxels,
+                                      uint *totalVerts, uint *d_compVoxelArray, uint maxVerts,
+                                      float *vol_one, float isovalue1, float isovalue2) {
+        using namespace gpucad_compat;
+        ensure_init();
+        check(gpucad_compute_isosurface_latticeone(
+                  vol, (gc_float4 *)pos, (gc_float4 *)norm, isoValue, numVoxels, d_voxelVerts,
+                  d_voxelVertsScan, d_voxelOccupied, d_voxelOccupiedScan, u3(gridSize),
+                  u3(gridSizeShift), u3(gridSizeMask), f3(voxelSize), f3(gridcenter),
+                  activeVoxels, totalVerts, d_compVoxelArray, maxVerts, vol_one, isovalue1,
+                  isovalue2),
+              "computeIsosurface_latticeone");
+    }
+};
+
+class Modelling {
+public:
+    static float a;
+    static float b;
+
+    Modelling() = default;
+    Modelling(int, int, int) {}
+
+    void distance_from_line(float *data_1, float3 center, float3 axis, float radius_1,
+                            float thickness_radial, float thickness_axial, int Nx, int Ny, int Nz,
+                            float dx, float dy, float dz, bool onetime) {
+        gpucad_compat::check(
+            gpucad_distance_from_line(data_1, gpucad_compat::f3(center), gpucad_compat::f3(axis),
+                                      radius_1, thickness_radial, thickness_axial, Nx, Ny, Nz, dx,
+                                      dy, dz, onetime ? 1 : 0),
+            "distance_from_line");
+    }
+
+    void sphere_with_center(float *data_1, float3 center, float radius_1, float thickness_wall,
+                            int Nx, int Ny, int Nz, float dx, float dy, float dz, bool onetime) {
+        gpucad_compat::check(gpucad_sphere_with_center(data_1, gpucad_compat::f3(center), radius_1,
+                                                       thickness_wall, Nx, Ny, Nz, dx, dy, dz,
+                                                       onetime ? 1 : 0),
+                             "sphere_with_center");
+    }
+
+    void cuboid(float *data_1, float3 center, float3 angles, float x_width, float y_width,
+                float z_width, int Nx, int Ny, int Nz, float dx, float dy, float dz) {
+        gpucad_compat::check(gpucad_cuboid(data_1, gpucad_compat::f3(center),
+                                           gpucad_compat::f3(angles), x_width, y_width, z_width,
+                                           Nx, Ny, Nz, dx, dy, dz),
+                             "cuboid");
+    }
+
+    void cuboid_shell(float *data_1, float3 center, float3 angles, float x_width, float y_width,
+                      float z_width, float thickness, int Nx, int Ny, int Nz, float dx, float dy,
+                      float dz) {
+        gpucad_compat::check(gpucad_cuboid_shell(data_1, gpucad_compat::f3(center),
+                                                 gpucad_compat::f3(angles), x_width, y_width,
+                                                 z_width, thickness, Nx, Ny, Nz, dx, dy, dz),
+                             "cuboid_shell");
+    }
+
+    void torus_with_center(float *data_1, float3 center, float3 angles, float torus_radius,
+                           float torus_circle_radius, int Nx, int Ny, int Nz, float dx, float dy,
+                           float dz) {
+        gpucad_compat::check(gpucad_torus_with_center(data_1, gpucad_compat::f3(center),
+                                                      gpucad_compat::f3(angles), torus_radius,
+                                                      torus_circle_radius, Nx, Ny, Nz, dx, dy,
+                                                      dz),
+                             "torus_with_center");
+    }
+
+    void cone_with_base_radius_height(float *data_1, float3 center, float3 angles,
+                                      float base_radius, float cone_height, int Nx, int Ny,
+                                      int Nz, float dx, float dy, float dz) {
+        gpucad_compat::check(
+            gpucad_cone_with_base_radius_height(data_1, gpucad_compat::f3(center),
+                                                gpucad_compat::f3(angles), base_radius,
+                                                cone_height, Nx, Ny, Nz, dx, dy, dz),
+            "cone_with_base_radius_height");
+    }
+
+    void cone_frustum(float *data_1, float3 center, float3 angles, float top_radius,
+                      float bottom_radius, float cone_frustum_height, int Nx, int Ny, int Nz,
+                      float dx, float dy, float dz) {
+        gpucad_compat::check(gpucad_cone_frustum(data_1, gpucad_compat::f3(center),
+                                                 gpucad_compat::f3(angles), top_radius,
+                                                 bottom_radius, cone_frustum_height, Nx, Ny, Nz,
+                                                 dx, dy, dz),
+                             "cone_frustum");
+    }
+
+    void pyramid_frustum(float *data_1, float3 center, float3 angles, float x_width_base,
+                         float x_width_top, float y_height, float z_width_base,
+                         float z_width_top, int Nx, int Ny, int Nz, float dx, float dy,
+                         float dz) {
+        gpucad_compat::check(gpucad_pyramid_frustum(data_1, gpucad_compat::f3(center),
+                                                    gpucad_compat::f3(angles), x_width_base,
+                                                    x_width_top, y_height, z_width_base,
+                                                    z_width_top, Nx, Ny, Nz, dx, dy, dz),
+                             "pyramid_frustum");
+    }
+};
+
+inline float Modelling::a = 0.0f;
+inline float Modelling::b = 0.0f;
+
+class Fft_lattice {
+public:
+    void create_lattice(float *d_latticevol, uint NX, uint NY, uint NZ, uint size,
+                        uint lattice_type_index) {
+        gpucad_compat::ensure_init();
+        gpucad_compat::check(
+            gpucad_create_lattice(d_latticevol, NX, NY, NZ, size, lattice_type_index),
+            "create_lattice");
+    }
+};
+
+class Gratings {
+public:
+    int NX = 0, NY = 0, NZ = 0;
+
+    void GPU_buffer_normalise_four(float *dataone, float *datatwo, float *datathree, size_t size,
+                                   int Nx, int Ny, int Nz, float isoval_1, float isoval_2) {
+        gpucad_compat::check(gpucad_normalise_four(dataone, datatwo, datathree, size, Nx, Ny, Nz,
+                                                   isoval_1, isoval_2),
+                             "GPU_buffer_normalise_four");
+    }
+
+    void primitive_field(grid_points *primitive_field, float *primitive_active, float *isosurf,
+                         float isoval, bool fixed, bool active, int NX_, int NY_, int NZ_) {
+        (void)isoval;
+        if (fixed)
+            gpucad_compat::check(
+                gpucad_primitive_field_fixed(primitive_field, isosurf, NX_, NY_, NZ_),
+                "primitive_field");
+        else if (active)
+            gpucad_compat::check(
+                gpucad_primitive_field_dynamic(primitive_active, isosurf, NX_, NY_, NZ_),
+                "primitive_field");
+    }
+};
+
+class File_output {
+public:
+    void file_write_obj(float4 *d_pos, uint totalVerts, const char *filename) {
+        gpucad_compat::check(gpucad_file_write_obj((const gc_float4 *)d_pos, totalVerts, filename),
+                             "file_write_obj");
+    }
+};
