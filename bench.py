@@ -51,6 +51,10 @@ def parse():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--calib", action="store_true",
                     help="also run a 512 MiB device copy (counter calibration)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched form")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served form")
+    ap.add_argument("--batch-runs", type=int, default=10)
+    ap.add_argument("--serve-requests", type=int, default=1000)
     return ap.parse_args()
 
 
@@ -125,6 +129,108 @@ def cpu_baseline(p, slab, threads, budget_s=12.0):
                   "%.1f s total): oracle field %.3f s + indexed extraction %.3f s per slab"
                   % (len(runs), slab + 1, p.nx, p.ny, p.nz, pts, spent, tf, te),
     }
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[k]
+
+
+def bench_batched(runs):
+    """256 grids of uneven sizes and types: one graph replay per run (field +
+    extraction of every grid) against 256 plain per-grid calls."""
+    grids = CF.batched_grids(256)
+    pts = sum(g.npoints() for g in grids)
+    t0 = time.perf_counter()
+    b = gc.Batch(grids, use_graph=True)
+    create_s = time.perf_counter() - t0
+    stream = torch.cuda.Stream()
+    b.run(stream)
+    torch.cuda.synchronize()
+    res = b.results()
+    tris = sum(r["n_triangles"] for r in res)
+    verts = sum(r["n_vertices"] for r in res)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(runs):
+        b.run(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / runs
+    tuning = b.tuning()
+    b.close()
+    # plain: one gpucad_lattice_field + gpucad_extract (synchronous counts) per grid
+    gc.plain_run_trace(grids[:8], [0.0] * 8)  # warm-up
+    _, plain_s = gc.plain_run_trace(grids, [0.0] * len(grids))
+    plain_ms = plain_s * 1e3
+    return {
+        "workload": "256 grids, per-axis sizes U{32,40,..,256}, type U{gyroid, Schwarz-P, "
+                    "diamond}, period U[8,48] voxels, sheet thickness U[0.1,0.5], seed 0xBA7C",
+        "points": pts,
+        "n_vertices": verts,
+        "n_triangles": tris,
+        "ms_per_run": round(ms, 4),
+        "gvoxels_per_s": round(pts / (ms * 1e-3) / 1e9, 3),
+        "mtri_per_s": round(tris / (ms * 1e-3) / 1e6, 1),
+        "plain_ms": round(plain_ms, 3),
+        "plain_gvoxels_per_s": round(pts / (plain_ms * 1e-3) / 1e9, 3),
+        "speedup_vs_plain": round(plain_ms / ms, 2),
+        "graph": True,
+        "tuning": {k: tuning[k] for k in ("brick_x", "brick_lz", "waves_per_cu")},
+        "create_s": round(create_s, 3),
+    }
+
+
+def bench_served(n_req):
+    """Lattice-edit requests (cubic edge lognormal median 96, cap 384; random
+    lattice + sphere/box boolean): saturation throughput (all requests queued
+    at t=0) and p50/p99 latency under Poisson arrivals at fractions of it,
+    for the server (continuous batching, 64 GB budget) and for plain
+    per-request calls."""
+    reqs = CF.served_requests(n_req)
+    pts = sum(r.npoints() for r in reqs)
+    budget = 64 << 30
+    srv = gc.Server(budget)
+    srv.run_trace(reqs[:32], [0.0] * 32)  # warm-up
+    lat0, wall0 = srv.run_trace(reqs, [0.0] * n_req)
+    sat_rps = n_req / wall0
+    out = {
+        "workload": "%d requests, cubic edge round8(min(384, lognormal(median 96, sigma 0.5))), "
+                    "gyroid/Schwarz-P/diamond sheet + sphere or box boolean, seed 0x5E7E; "
+                    "64 GB admission budget" % n_req,
+        "points": pts,
+        "saturation": {"requests_per_s": round(sat_rps, 1),
+                       "gvoxels_per_s": round(pts / wall0 / 1e9, 3)},
+        "rates": [],
+    }
+    gc.plain_run_trace(reqs[:16], [0.0] * 16)  # warm-up
+    plat0, pwall0 = gc.plain_run_trace(reqs, [0.0] * n_req)
+    plain_rps = n_req / pwall0
+    out["plain_saturation"] = {"requests_per_s": round(plain_rps, 1),
+                               "gvoxels_per_s": round(pts / pwall0 / 1e9, 3)}
+    out["throughput_vs_plain"] = round(sat_rps / plain_rps, 2)
+    for frac in (0.25, 0.5, 0.8):
+        rate = frac * sat_rps
+        arr = CF.poisson_arrivals(n_req, rate)
+        lat, wall = srv.run_trace(reqs, arr)
+        row = {"rate_rps": round(rate, 1), "fraction_of_saturation": frac,
+               "p50_ms": round(1e3 * percentile(lat, 50), 3),
+               "p99_ms": round(1e3 * percentile(lat, 99), 3),
+               "achieved_rps": round(n_req / wall, 1)}
+        if rate < plain_rps:  # the plain path can only be compared where it keeps up
+            plat, pwall = gc.plain_run_trace(reqs, arr)
+            row["plain_p50_ms"] = round(1e3 * percentile(plat, 50), 3)
+            row["plain_p99_ms"] = round(1e3 * percentile(plat, 99), 3)
+        out["rates"].append(row)
+    st = srv.stats()
+    out["server"] = {"groups": st["groups"], "mean_group_size": round(st["mean_group_size"], 2),
+                     "peak_bytes_in_use": st["peak_bytes_in_use"], "budget_bytes": budget}
+    srv.close()
+    return out
 
 
 def main():
@@ -203,7 +309,16 @@ def main():
         "n_triangles": counts[1],
         "wall_s": round(wall, 3),
     }
+    del pipe
+    torch.cuda.empty_cache()
+    if not args.no_batch:
+        print("bench: batched form", file=sys.stderr, flush=True)
+        rec["batched"] = bench_batched(args.batch_runs)
+    if not args.no_serve:
+        print("bench: served form", file=sys.stderr, flush=True)
+        rec["served"] = bench_served(args.serve_requests)
     if not args.no_cpu:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
         rec["cpu_baseline"] = cpu_baseline(p, args.cpu_slab, args.cpu_threads, args.cpu_budget)
     print(json.dumps(rec), flush=True)
 

@@ -99,6 +99,78 @@ class MeshCounts(C.Structure):
     _fields_ = [("n_vertices", C.c_uint64), ("n_triangles", C.c_uint64)]
 
 
+class BatchOptions(C.Structure):
+    _fields_ = [
+        ("iso", C.c_float),
+        ("brick_x", C.c_int32),
+        ("brick_lz", C.c_int32),
+        ("waves_per_cu", C.c_int32),
+        ("use_graph", C.c_int32),
+        ("unit_domain", C.c_int32),
+        ("vert_capacity", C.c_uint64),
+        ("tri_capacity", C.c_uint64),
+    ]
+
+
+class BatchGridResult(C.Structure):
+    _fields_ = [
+        ("field_offset", C.c_uint64),
+        ("vert_offset", C.c_uint64),
+        ("n_vertices", C.c_uint64),
+        ("tri_offset", C.c_uint64),
+        ("n_triangles", C.c_uint64),
+        ("status", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+
+class BatchBuffers(C.Structure):
+    _fields_ = [
+        ("fields", C.c_void_p),
+        ("field_floats", C.c_uint64),
+        ("vertices", C.c_void_p),
+        ("vert_capacity", C.c_uint64),
+        ("triangles", C.c_void_p),
+        ("tri_capacity", C.c_uint64),
+    ]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [
+        ("budget_bytes", C.c_uint64),
+        ("max_batch", C.c_int32),
+        ("slots", C.c_int32),
+        ("iso", C.c_float),
+        ("brick_x", C.c_int32),
+        ("brick_lz", C.c_int32),
+        ("waves_per_cu", C.c_int32),
+    ]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [
+        ("ticket", C.c_uint64),
+        ("status", C.c_int32),
+        ("pad", C.c_int32),
+        ("field", C.c_void_p),
+        ("vertices", C.c_void_p),
+        ("triangles", C.c_void_p),
+        ("n_vertices", C.c_uint64),
+        ("n_triangles", C.c_uint64),
+        ("submit_s", C.c_double),
+        ("done_s", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+        ("groups", C.c_uint64), ("bytes_in_use", C.c_uint64),
+        ("peak_bytes_in_use", C.c_uint64), ("budget_bytes", C.c_uint64),
+        ("mean_group_size", C.c_double),
+    ]
+
+
 _vp, _u32, _i32, _u64, _f = C.c_void_p, C.c_uint32, C.c_int32, C.c_uint64, C.c_float
 
 # signatures of every exported entry point (argtypes, restype)
@@ -115,6 +187,23 @@ SIGNATURES = {
     "gpucad_lattice_field_lz": ([C.POINTER(LatticeParams), _vp, _vp, _f, _i32, _vp], C.c_int),
     "gpucad_extract_workspace_bytes": ([_i32, _i32, _i32], _u64),
     "gpucad_extract": ([C.POINTER(ExtractArgs), C.POINTER(MeshCounts), _vp], C.c_int),
+    "gpucad_batch_create": ([C.POINTER(LatticeParams), _i32, C.POINTER(BatchOptions),
+                             C.POINTER(_vp)], C.c_int),
+    "gpucad_batch_run": ([_vp, _vp], C.c_int),
+    "gpucad_batch_results": ([_vp, C.POINTER(BatchGridResult), _i32], C.c_int),
+    "gpucad_batch_buffers": ([_vp, C.POINTER(BatchBuffers)], C.c_int),
+    "gpucad_batch_tuning": ([_vp, C.POINTER(BatchOptions)], C.c_int),
+    "gpucad_batch_destroy": ([_vp], C.c_int),
+    "gpucad_server_create": ([C.POINTER(ServerOptions), C.POINTER(_vp)], C.c_int),
+    "gpucad_server_submit": ([_vp, C.POINTER(LatticeParams), C.POINTER(_u64)], C.c_int),
+    "gpucad_server_poll": ([_vp, C.POINTER(RequestResult), _i32, C.POINTER(_i32)], C.c_int),
+    "gpucad_server_release": ([_vp, _u64], C.c_int),
+    "gpucad_server_stats": ([_vp, C.POINTER(ServerStats)], C.c_int),
+    "gpucad_server_destroy": ([_vp], C.c_int),
+    "gpucad_server_run_trace": ([_vp, C.POINTER(LatticeParams), C.POINTER(C.c_double), _i32,
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+    "gpucad_plain_run_trace": ([C.POINTER(LatticeParams), C.POINTER(C.c_double), _i32, _f,
+                                C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "gpucad_create_lattice": ([_vp, _u32, _u32, _u32, _u32, _u32], C.c_int),
     "gpucad_sphere_with_center": ([_vp, Float3, _f, _f, _i32, _i32, _i32, _f, _f, _f, _i32], C.c_int),
     "gpucad_cuboid": ([_vp, Float3, Float3, _f, _f, _f, _i32, _i32, _i32, _f, _f, _f], C.c_int),
@@ -276,3 +365,168 @@ def u3(x, y, z):
 
 def f3(x, y, z):
     return Float3(x, y, z)
+
+
+# ---------------------------------------------------------------------------
+# batched form
+# ---------------------------------------------------------------------------
+def _device_view(addr, count, dtype, device):
+    """A torch tensor over `count` elements of library-owned device memory
+    (valid while the owner lives)."""
+    if count == 0:
+        return torch.empty(0, dtype=dtype, device=device)
+
+    class _Holder:  # wraps the memory through the CUDA array interface
+        __cuda_array_interface__ = {
+            "shape": (count,),
+            "typestr": {torch.float32: "<f4", torch.int32: "<i4", torch.int64: "<i8"}[dtype],
+            "data": (addr, False),
+            "version": 2,
+        }
+    return torch.as_tensor(_Holder(), device=device)
+
+
+class Batch:
+    """Many lattice grids evaluated and meshed per run (gpucad_batch_*).
+
+    grids: list of LatticeParams.  Options left at 0 are autotuned.
+    """
+
+    def __init__(self, grids, iso=0.0, brick_x=0, brick_lz=0, waves_per_cu=0, use_graph=True,
+                 unit_domain=True, vert_capacity=0, tri_capacity=0):
+        arr = (LatticeParams * len(grids))(*grids)
+        o = BatchOptions()
+        o.iso, o.brick_x, o.brick_lz, o.waves_per_cu = iso, brick_x, brick_lz, waves_per_cu
+        o.use_graph, o.unit_domain = int(bool(use_graph)), int(bool(unit_domain))
+        o.vert_capacity, o.tri_capacity = vert_capacity, tri_capacity
+        h = C.c_void_p()
+        check(lib.gpucad_batch_create(arr, len(grids), C.byref(o), C.byref(h)))
+        self._h = h
+        self.n = len(grids)
+        self.grids = list(grids)
+        b = BatchBuffers()
+        check(lib.gpucad_batch_buffers(self._h, C.byref(b)))
+        self.buffers = b
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.fields = _device_view(b.fields, b.field_floats, torch.float32, dev)
+        self.vertices = _device_view(b.vertices, 3 * b.vert_capacity, torch.float32,
+                                     dev).view(-1, 3)
+        self.triangles = _device_view(b.triangles, 3 * b.tri_capacity, torch.int32,
+                                      dev).view(-1, 3)
+
+    def tuning(self):
+        o = BatchOptions()
+        check(lib.gpucad_batch_tuning(self._h, C.byref(o)))
+        return dict(brick_x=o.brick_x, brick_lz=o.brick_lz, waves_per_cu=o.waves_per_cu,
+                    vert_capacity=o.vert_capacity, tri_capacity=o.tri_capacity)
+
+    def run(self, stream=None):
+        check(lib.gpucad_batch_run(self._h, stream_handle(stream)))
+
+    def results(self, allow_overflow=False):
+        res = (BatchGridResult * self.n)()
+        st = lib.gpucad_batch_results(self._h, res, self.n)
+        if st != OK and not (allow_overflow and st == E_OVERFLOW):
+            check(st)
+        return [dict(field_offset=r.field_offset, vert_offset=r.vert_offset,
+                     n_vertices=r.n_vertices, tri_offset=r.tri_offset,
+                     n_triangles=r.n_triangles, status=r.status) for r in res]
+
+    def field(self, i, res=None):
+        p = self.grids[i]
+        off = (res or self.results())[i]["field_offset"]
+        return self.fields[off:off + p.npoints()]
+
+    def mesh(self, i, res=None):
+        r = (res or self.results())[i]
+        v = self.vertices[r["vert_offset"]:r["vert_offset"] + r["n_vertices"]]
+        t = self.triangles[r["tri_offset"]:r["tri_offset"] + r["n_triangles"]]
+        return Mesh(v, t, r["n_vertices"], r["n_triangles"])
+
+    def close(self):
+        if self._h:
+            check(lib.gpucad_batch_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# served form
+# ---------------------------------------------------------------------------
+class Server:
+    """Request loop with continuous batching under a device-memory budget
+    (gpucad_server_*)."""
+
+    def __init__(self, budget_bytes, max_batch=0, slots=0, iso=0.0, brick_x=0, brick_lz=0,
+                 waves_per_cu=0):
+        o = ServerOptions()
+        o.budget_bytes, o.max_batch, o.slots, o.iso = budget_bytes, max_batch, slots, iso
+        o.brick_x, o.brick_lz, o.waves_per_cu = brick_x, brick_lz, waves_per_cu
+        h = C.c_void_p()
+        check(lib.gpucad_server_create(C.byref(o), C.byref(h)))
+        self._h = h
+        self.device = torch.device("cuda", torch.cuda.current_device())
+
+    def submit(self, params):
+        t = C.c_uint64()
+        check(lib.gpucad_server_submit(self._h, C.byref(params), C.byref(t)))
+        return t.value
+
+    def poll(self, max_results=64):
+        out = (RequestResult * max_results)()
+        n = C.c_int32()
+        check(lib.gpucad_server_poll(self._h, out, max_results, C.byref(n)))
+        return [out[i] for i in range(n.value)]
+
+    def release(self, ticket):
+        check(lib.gpucad_server_release(self._h, ticket))
+
+    def stats(self):
+        s = ServerStats()
+        check(lib.gpucad_server_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def result_arrays(self, r, params):
+        """Host copies (field, vertices, triangles) of a finished request."""
+        f = _device_view(r.field, params.npoints(), torch.float32, self.device).cpu().numpy()
+        v = _device_view(r.vertices, 3 * r.n_vertices, torch.float32,
+                         self.device).cpu().numpy().reshape(-1, 3)
+        t = _device_view(r.triangles, 3 * r.n_triangles, torch.int32,
+                         self.device).cpu().numpy().reshape(-1, 3)
+        return f, v, t
+
+    def run_trace(self, requests, arrivals):
+        n = len(requests)
+        reqs = (LatticeParams * n)(*requests)
+        arr = (C.c_double * n)(*arrivals)
+        lat = (C.c_double * n)()
+        wall = C.c_double()
+        check(lib.gpucad_server_run_trace(self._h, reqs, arr, n, lat, C.byref(wall)))
+        return list(lat), wall.value
+
+    def close(self):
+        if self._h:
+            check(lib.gpucad_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plain_run_trace(requests, arrivals, iso=0.0):
+    """The same requests through plain per-request calls, one at a time."""
+    n = len(requests)
+    reqs = (LatticeParams * n)(*requests)
+    arr = (C.c_double * n)(*arrivals)
+    lat = (C.c_double * n)()
+    wall = C.c_double()
+    check(lib.gpucad_plain_run_trace(reqs, arr, n, iso, lat, C.byref(wall)))
+    return list(lat), wall.value

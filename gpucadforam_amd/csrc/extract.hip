@@ -19,29 +19,36 @@
 // Launches (one stream, no host round trip, no inter-block waits):
 //   signs : ballot of f < iso per 64 points; skipped when the field kernel
 //           already produced the words
-//   count : blocks of 256 words: crossing edges per word (vertices), triangles
-//           per wave (case of each active cell), block sums
+//   count : blocks of 256 words: triangles per wave (case of each active
+//           cell), vertex and triangle sums per block
 //   scan  : one workgroup scans the block sums (64-bit), writes the totals
-//   vert  : block scan of the word counts -> per-word first vertex id (kept
-//           for `tri`); the block's crossing edges are compacted into an LDS
-//           list in canonical order and emitted one lane per edge (loads of
-//           f0 and f1, interpolation); the block's vertices form one
+//           and the vertex/triangle range of each grid
+//   vert  : block scan of the word vertex counts -> per-word first vertex
+//           number (kept for `tri`); the block's crossing edges are compacted
+//           into an LDS list in canonical order and emitted one lane per edge
+//           (loads of f0 and f1, interpolation); the block's vertices form one
 //           contiguous range, stored coalesced
-//   tri   : one lane per active cell computes the 12 edge vertex ids from
+//   tri   : one lane per active cell computes the 12 edge vertex numbers from
 //           popcounts of the crossing words of the four rows touching the cell
 //           and writes its triangles; a wave's triangles form one contiguous
 //           range starting at the wave's scanned base
-#include "gc_common.hpp"
+//
+// The kernels see grids as segments: a grid's words occupy a contiguous range
+// of global blocks.  The one-grid entry point launches one block per block of
+// the grid; the many-grid form (batched and served) runs persistent
+// workgroups that take global blocks from a device queue (an atomic counter)
+// and find each block's grid in a block -> grid table.  Both run the same
+// block code, so a grid's mesh does not depend on what it is launched with.
+#include "extract_types.hpp"
 
 namespace gc {
 namespace {
 
-constexpr int kBlock = 256;    // words per block (4 waves)
 constexpr int kListCap = 512;  // per-wave LDS list entries per round
 constexpr int kVertCap = 4096; // per-block LDS edge-list entries per round
 
 struct ExtractLayout {
-    uint64_t signs_off, vcnt_off, base_off, twave_off, bsum_off, ctl_off, bytes;
+    uint64_t signs_off, base_off, twave_off, bsum_off, ctl_off, res_off, bytes;
     uint64_t words, nb;
 };
 
@@ -54,16 +61,15 @@ ExtractLayout layout(int nx, int ny, int nz) {
     uint64_t off = 0;
     L.signs_off = off;
     off = align256(off + L.words * 8);
-    L.vcnt_off = off;
-    off = align256(off + L.words * 4);
     L.base_off = off;
-    off = align256(off + L.words * 4);
+    off = align256(off + L.nb * kBlock * 4);
     L.twave_off = off;
     off = align256(off + L.nb * 4 * 4);
     L.bsum_off = off;
     off = align256(off + L.nb * 16);
-    L.ctl_off = off;  // [0] n_vertices [1] n_triangles [2] vertex ids overflow 32 bits
-    off = align256(off + 64);
+    L.ctl_off = off;  // [0] n_vertices [1] n_triangles [2] vertex numbers overflow 32 bits
+    L.res_off = off + 64;
+    off = align256(off + 64 + sizeof(SegResult));
     L.bytes = off;
     return L;
 }
@@ -82,11 +88,9 @@ __device__ __forceinline__ uint64_t mask_below(int64_t n) {
 __device__ __forceinline__ uint32_t bit_at(uint64_t w, uint32_t b) {
     return (uint32_t)(w >> b) & 1u;
 }
-__device__ __forceinline__ uint32_t mbcnt(uint64_t w) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(w >> 32),
-                                     __builtin_amdgcn_mbcnt_lo((uint32_t)w, 0u));
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
-__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
 __device__ __forceinline__ uint32_t gather32(uint32_t v, uint32_t src) {
     return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
@@ -97,9 +101,6 @@ __device__ __forceinline__ uint64_t gather64(uint64_t v, uint32_t src) {
 }
 __device__ __forceinline__ uint32_t readlane32(uint32_t v, uint32_t l) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
-}
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
-    return ((uint64_t)readlane32((uint32_t)(v >> 32), l) << 32) | readlane32((uint32_t)v, l);
 }
 
 // inclusive scan over the 64 lanes of a wave
@@ -121,14 +122,6 @@ struct T3 {
     uint32_t a, b, c;
 };
 
-// n / d for n < 2^32 through a double reciprocal and one correction step
-struct FastDiv {
-    uint32_t d;
-    double inv;
-};
-
-inline FastDiv make_div(uint32_t d) { return FastDiv{d, 1.0 / (double)d}; }
-
 __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) {
     uint32_t q = (uint32_t)((double)n * f.inv);
     const int64_t r = (int64_t)n - (int64_t)q * f.d;
@@ -136,12 +129,6 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) {
     else if (r >= (int64_t)f.d) ++q;
     return q;
 }
-
-struct Grid {
-    int nx, ny, nz, W;
-    uint64_t words;
-    FastDiv divW, divNy;
-};
 
 __device__ __forceinline__ uint64_t sword(const uint64_t *s, const Grid &g, int y, int z, int w) {
     return (y < g.ny && z < g.nz && w < g.W) ? s[((uint64_t)z * g.ny + y) * g.W + w] : 0ull;
@@ -197,6 +184,49 @@ __device__ __forceinline__ uint64_t active_cells(uint64_t a, uint64_t sa, uint64
 }
 
 // ---------------------------------------------------------------------------
+// grids and block scheduling
+// ---------------------------------------------------------------------------
+struct OneGrid {
+    Seg s;
+    __device__ __forceinline__ uint32_t seg_of(uint32_t) const { return 0; }
+    __device__ __forceinline__ Seg get(uint32_t) const { return s; }
+};
+
+struct ManyGrids {
+    const Seg *__restrict__ segs;
+    const uint32_t *__restrict__ blk2seg;
+    __device__ __forceinline__ uint32_t seg_of(uint32_t b) const {
+        return (uint32_t)__builtin_amdgcn_readfirstlane((int)blk2seg[b]);
+    }
+    __device__ __forceinline__ Seg get(uint32_t i) const { return segs[i]; }
+};
+
+// counter == NULL: the workgroup runs block blockIdx.x.  Otherwise the
+// workgroup is persistent and takes blocks from the counter until it passes
+// n; every workgroup leaves after one failed take.
+struct Queue {
+    uint32_t *counter;
+    uint32_t n;
+};
+
+template <class F>
+__device__ __forceinline__ void for_each_block(const Queue &q, F &&f) {
+    if (!q.counter) {
+        f((uint32_t)blockIdx.x);
+        return;
+    }
+    __shared__ uint32_t s_item;
+    for (;;) {
+        __syncthreads();  // the previous block is done with LDS
+        if (threadIdx.x == 0) s_item = atomicAdd(q.counter, 1u);
+        __syncthreads();
+        const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_item);
+        if (b >= q.n) break;
+        f(b);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // signs
 // ---------------------------------------------------------------------------
 constexpr int kSignZ = 8;
@@ -238,83 +268,96 @@ __device__ __forceinline__ void fill_list(uint16_t *list, uint64_t act, uint32_t
 // ---------------------------------------------------------------------------
 // count
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock)
-count_kernel(const uint64_t *__restrict__ signs, uint32_t *__restrict__ vcnt,
-             uint32_t *__restrict__ twave, uint64_t *__restrict__ bsum,
-             const uint8_t *__restrict__ nverts, Grid g) {
+struct CountArgs {
+    const uint64_t *signs;  // sign buffer base
+    uint32_t *twave;
+    uint64_t *bsum;
+    const uint8_t *nverts;
+};
+
+template <class Src>
+__global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a, Queue q) {
     __shared__ uint8_t s_ntri[256];
     __shared__ uint16_t s_list[4][kListCap];
     __shared__ uint32_t s_sum[4][2];
-    s_ntri[threadIdx.x] = nverts[threadIdx.x] / 3;
+    s_ntri[threadIdx.x] = a.nverts[threadIdx.x] / 3;
     __syncthreads();
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint64_t k = (uint64_t)blockIdx.x * kBlock + tid;
-    uint64_t a = 0, sa = 0, b = 0, sb = 0, c = 0, sc = 0, d = 0, sd = 0, act = 0;
-    uint32_t vc = 0;
-    if (k < g.words) {
-        const WordPos p = word_pos(k, g);
-        const bool has_y = p.y + 1 < g.ny, has_z = p.z + 1 < g.nz;
-        a = sword(signs, g, p.y, p.z, p.w);
-        const uint64_t an = sword(signs, g, p.y, p.z, p.w + 1);
-        b = sword(signs, g, p.y + 1, p.z, p.w);
-        const uint64_t bn = sword(signs, g, p.y + 1, p.z, p.w + 1);
-        c = sword(signs, g, p.y, p.z + 1, p.w);
-        const uint64_t cn = sword(signs, g, p.y, p.z + 1, p.w + 1);
-        sa = shift1(a, an);
-        sb = shift1(b, bn);
-        sc = shift1(c, cn);
-        const uint64_t vpt = mask_below((int64_t)g.nx - 64 * p.w);
-        const uint64_t ved = mask_below((int64_t)g.nx - 1 - 64 * p.w);
-        const uint64_t ex = (a ^ sa) & ved;
-        const uint64_t ey = has_y ? (a ^ b) & vpt : 0ull;
-        const uint64_t ez = has_z ? (a ^ c) & vpt : 0ull;
-        vc = popc64(ex) + popc64(ey) + popc64(ez);
-        if (has_y && has_z) {
-            d = sword(signs, g, p.y + 1, p.z + 1, p.w);
-            sd = shift1(d, sword(signs, g, p.y + 1, p.z + 1, p.w + 1));
-            act = active_cells(a, sa, b, sb, c, sc, d, sd, ved);
+    for_each_block(q, [&](uint32_t gb) {
+        const Seg s = src.get(src.seg_of(gb));
+        const Grid &g = s.g;
+        const uint64_t *signs = a.signs + s.sign_off;
+        const uint64_t k = (uint64_t)(gb - s.block_off) * kBlock + tid;
+        uint64_t wa = 0, sa = 0, wb = 0, sb = 0, wc = 0, sc = 0, wd = 0, sd = 0, act = 0;
+        uint32_t vc = 0;
+        if (k < g.words) {
+            const WordPos p = word_pos(k, g);
+            const bool has_y = p.y + 1 < g.ny, has_z = p.z + 1 < g.nz;
+            wa = sword(signs, g, p.y, p.z, p.w);
+            const uint64_t an = sword(signs, g, p.y, p.z, p.w + 1);
+            wb = sword(signs, g, p.y + 1, p.z, p.w);
+            const uint64_t bn = sword(signs, g, p.y + 1, p.z, p.w + 1);
+            wc = sword(signs, g, p.y, p.z + 1, p.w);
+            const uint64_t cn = sword(signs, g, p.y, p.z + 1, p.w + 1);
+            sa = shift1(wa, an);
+            sb = shift1(wb, bn);
+            sc = shift1(wc, cn);
+            const uint64_t vpt = mask_below((int64_t)g.nx - 64 * p.w);
+            const uint64_t ved = mask_below((int64_t)g.nx - 1 - 64 * p.w);
+            const uint64_t ex = (wa ^ sa) & ved;
+            const uint64_t ey = has_y ? (wa ^ wb) & vpt : 0ull;
+            const uint64_t ez = has_z ? (wa ^ wc) & vpt : 0ull;
+            vc = popc64(ex) + popc64(ey) + popc64(ez);
+            if (has_y && has_z) {
+                wd = sword(signs, g, p.y + 1, p.z + 1, p.w);
+                sd = shift1(wd, sword(signs, g, p.y + 1, p.z + 1, p.w + 1));
+                act = active_cells(wa, sa, wb, sb, wc, sc, wd, sd, ved);
+            }
         }
-        vcnt[k] = vc;
-    }
-    const uint32_t nact = popc64(act);
-    const uint32_t aincl = wave_scan_incl(nact);
-    const uint32_t atot = readlane32(aincl, 63);
-    const uint32_t aex = aincl - nact;
-    uint16_t *list = s_list[wave];
-    uint32_t tsum = 0;  // per-lane partial triangle count of the wave
-    for (uint32_t r0 = 0; r0 < atot; r0 += kListCap) {
-        fill_list(list, act, nact, aex, r0, lane);
-        const uint32_t n = min((uint32_t)kListCap, atot - r0);
-        for (uint32_t e0 = 0; e0 < n; e0 += 64) {
-            const uint32_t e = e0 + lane;
-            const uint32_t ent = e < n ? list[e] : 0u;
-            const uint32_t j = ent & 63, x = ent >> 6;
-            const uint32_t cube = cube_at(gather64(a, j), gather64(sa, j), gather64(b, j),
-                                          gather64(sb, j), gather64(c, j), gather64(sc, j),
-                                          gather64(d, j), gather64(sd, j), x);
-            if (e < n) tsum += s_ntri[cube];
+        const uint32_t nact = popc64(act);
+        const uint32_t aincl = wave_scan_incl(nact);
+        const uint32_t atot = readlane32(aincl, 63);
+        const uint32_t aex = aincl - nact;
+        uint16_t *list = s_list[wave];
+        uint32_t tsum = 0;  // per-lane partial triangle count of the wave
+        for (uint32_t r0 = 0; r0 < atot; r0 += kListCap) {
+            fill_list(list, act, nact, aex, r0, lane);
+            const uint32_t n = min((uint32_t)kListCap, atot - r0);
+            for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+                const uint32_t e = e0 + lane;
+                const uint32_t ent = e < n ? list[e] : 0u;
+                const uint32_t j = ent & 63, x = ent >> 6;
+                const uint32_t cube = cube_at(gather64(wa, j), gather64(sa, j), gather64(wb, j),
+                                              gather64(sb, j), gather64(wc, j), gather64(sc, j),
+                                              gather64(wd, j), gather64(sd, j), x);
+                if (e < n) tsum += s_ntri[cube];
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_wave_barrier();
-    }
-    const uint32_t vw = readlane32(wave_scan_incl(vc), 63);
-    const uint32_t tw = readlane32(wave_scan_incl(tsum), 63);
-    if (lane == 0) {
-        s_sum[wave][0] = vw;
-        s_sum[wave][1] = tw;
-        twave[4 * (uint64_t)blockIdx.x + wave] = tw;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        bsum[2 * blockIdx.x] = (uint64_t)s_sum[0][0] + s_sum[1][0] + s_sum[2][0] + s_sum[3][0];
-        bsum[2 * blockIdx.x + 1] = (uint64_t)s_sum[0][1] + s_sum[1][1] + s_sum[2][1] + s_sum[3][1];
-    }
+        const uint32_t vw = readlane32(wave_scan_incl(vc), 63);
+        const uint32_t tw = readlane32(wave_scan_incl(tsum), 63);
+        if (lane == 0) {
+            s_sum[wave][0] = vw;
+            s_sum[wave][1] = tw;
+            a.twave[4 * (uint64_t)gb + wave] = tw;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            a.bsum[2 * (uint64_t)gb] =
+                (uint64_t)s_sum[0][0] + s_sum[1][0] + s_sum[2][0] + s_sum[3][0];
+            a.bsum[2 * (uint64_t)gb + 1] =
+                (uint64_t)s_sum[0][1] + s_sum[1][1] + s_sum[2][1] + s_sum[3][1];
+        }
+    });
 }
 
 // ---------------------------------------------------------------------------
-// scan of the block sums (one workgroup); ctl = totals and the overflow flag
+// scan of the block sums (one workgroup); ctl = totals and the overflow flag;
+// res = each grid's vertex and triangle range
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024)
-scan_blocks_kernel(uint64_t *bsum, uint64_t nb, uint64_t *ctl, uint64_t *d_counts) {
+scan_blocks_kernel(uint64_t *bsum, uint64_t nb, uint64_t *ctl, uint64_t *d_counts,
+                   const Seg *segs, uint32_t nseg, SegResult *res) {
     // thread t owns the strided entries t, t + 1024, ... so that every pass of
     // the loop is one coalesced load; the scan then runs over (pass, thread)
     __shared__ uint64_t sw_v[16], sw_t[16];
@@ -353,10 +396,29 @@ scan_blocks_kernel(uint64_t *bsum, uint64_t nb, uint64_t *ctl, uint64_t *d_count
         carry_t = tt;
         __syncthreads();
     }
+    // every thread holds the totals; the scanned sums are visible to the workgroup
+    int any_ovf = 0;
+    for (uint32_t i = t; i < nseg; i += 1024) {
+        const uint64_t b0 = segs ? segs[i].block_off : 0;
+        const uint64_t b1 = (segs && i + 1 < nseg) ? segs[i + 1].block_off : nb;
+        const uint64_t v0 = bsum[2 * b0], t0 = bsum[2 * b0 + 1];
+        const uint64_t v1 = b1 < nb ? bsum[2 * b1] : carry_v;
+        const uint64_t t1 = b1 < nb ? bsum[2 * b1 + 1] : carry_t;
+        SegResult r;
+        r.v_off = v0;
+        r.nv = v1 - v0;
+        r.t_off = t0;
+        r.nt = t1 - t0;
+        r.ovf = r.nv > 0xffffffffull ? 1u : 0u;
+        r.pad = 0;
+        res[i] = r;
+        any_ovf |= (int)r.ovf;
+    }
+    any_ovf = __syncthreads_or(any_ovf);
     if (t == 0) {
         ctl[CTL_NV] = carry_v;
         ctl[CTL_NT] = carry_t;
-        ctl[CTL_OVF] = carry_v > 0xffffffffull ? 1 : 0;
+        ctl[CTL_OVF] = any_ovf ? 1 : 0;
         if (d_counts) {
             d_counts[0] = carry_v;
             d_counts[1] = carry_t;
@@ -365,252 +427,273 @@ scan_blocks_kernel(uint64_t *bsum, uint64_t nb, uint64_t *ctl, uint64_t *d_count
 }
 
 // ---------------------------------------------------------------------------
-// vertices (+ per-word first vertex ids)
+// vertices (+ per-word first vertex numbers)
 // ---------------------------------------------------------------------------
 struct VertArgs {
-    const float *field;
-    const uint64_t *signs;
-    const uint32_t *vcnt;
-    const uint64_t *bsum;  // scanned block bases
-    const uint64_t *ctl;
+    const float *field;     // field buffer base
+    const uint64_t *signs;  // sign buffer base
+    const uint64_t *bsum;   // scanned block bases
+    const SegResult *res;
     uint32_t *base;
     float *vertices;
     uint64_t vcap;
-    Grid g;
-    float iso, ox, oy, oz, sx, sy, sz;
 };
 
-__global__ void __launch_bounds__(kBlock) vert_kernel(VertArgs a) {
+template <class Src>
+__global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a, Queue q) {
     __shared__ uint16_t s_list[kVertCap];
     __shared__ uint64_t s_p0[kBlock];   // linear index of the word's first point
-    __shared__ uint32_t s_xyz[kBlock];  // first x (16 bits) | y << 16 ... packed below
+    __shared__ uint32_t s_xyz[kBlock];  // first x | y << 16
     __shared__ uint32_t s_z[kBlock];
     __shared__ uint32_t s_wsum[4];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const Grid &g = a.g;
-    const uint64_t k = (uint64_t)blockIdx.x * kBlock + tid;
-    uint64_t ex = 0, ey = 0, ez = 0;
-    if (k < g.words) {
-        const WordPos p = word_pos(k, g);
-        const bool has_y = p.y + 1 < g.ny, has_z = p.z + 1 < g.nz;
-        const uint64_t w0 = sword(a.signs, g, p.y, p.z, p.w),
-                       w0n = sword(a.signs, g, p.y, p.z, p.w + 1);
-        const uint64_t vpt = mask_below((int64_t)g.nx - 64 * p.w);
-        const uint64_t ved = mask_below((int64_t)g.nx - 1 - 64 * p.w);
-        ex = (w0 ^ shift1(w0, w0n)) & ved;
-        ey = has_y ? (w0 ^ sword(a.signs, g, p.y + 1, p.z, p.w)) & vpt : 0ull;
-        ez = has_z ? (w0 ^ sword(a.signs, g, p.y, p.z + 1, p.w)) & vpt : 0ull;
-        s_p0[tid] = ((uint64_t)p.z * g.ny + p.y) * g.nx + 64 * (uint64_t)p.w;
-        s_xyz[tid] = (uint32_t)(64 * p.w) | ((uint32_t)p.y << 16);
-        s_z[tid] = (uint32_t)p.z;
-    }
-    const uint32_t vc = popc64(ex) + popc64(ey) + popc64(ez);
-    const uint32_t incl = wave_scan_incl(vc);
-    if (lane == 63) s_wsum[wave] = incl;
-    __syncthreads();
-    uint32_t wbase = 0, vtot = 0;
-    for (uint32_t i = 0; i < 4; ++i) {
-        if (i < wave) wbase += s_wsum[i];
-        vtot += s_wsum[i];
-    }
-    const uint32_t vex = wbase + incl - vc;  // block-local first vertex of this word
-    const uint64_t vblock = a.bsum[2 * blockIdx.x];
-    if (k < g.words) a.base[k] = (uint32_t)(vblock + vex);
-    if (a.ctl[CTL_OVF]) return;  // ids do not fit 32 bits: nothing is written
-    const uint64_t plane = (uint64_t)g.nx * g.ny;
-    for (uint32_t r0 = 0; r0 < vtot; r0 += kVertCap) {
-        // this lane's edges, canonical order, that fall into the round's window
-        if (vc && vex < r0 + kVertCap && vex + vc > r0) {
-            uint64_t pts = ex | ey | ez;
-            uint32_t i = vex;
-            while (pts) {
-                const uint32_t b = (uint32_t)__builtin_ctzll(pts);
-                pts &= pts - 1;
-                const uint32_t m = bit_at(ex, b) | (bit_at(ey, b) << 1) | (bit_at(ez, b) << 2);
+    for_each_block(q, [&](uint32_t gb) {
+        const uint32_t si = src.seg_of(gb);
+        const Seg s = src.get(si);
+        const Grid &g = s.g;
+        const uint64_t *signs = a.signs + s.sign_off;
+        const uint64_t k = (uint64_t)(gb - s.block_off) * kBlock + tid;
+        uint64_t ex = 0, ey = 0, ez = 0;
+        if (k < g.words) {
+            const WordPos p = word_pos(k, g);
+            const bool has_y = p.y + 1 < g.ny, has_z = p.z + 1 < g.nz;
+            const uint64_t w0 = sword(signs, g, p.y, p.z, p.w),
+                           w0n = sword(signs, g, p.y, p.z, p.w + 1);
+            const uint64_t vpt = mask_below((int64_t)g.nx - 64 * p.w);
+            const uint64_t ved = mask_below((int64_t)g.nx - 1 - 64 * p.w);
+            ex = (w0 ^ shift1(w0, w0n)) & ved;
+            ey = has_y ? (w0 ^ sword(signs, g, p.y + 1, p.z, p.w)) & vpt : 0ull;
+            ez = has_z ? (w0 ^ sword(signs, g, p.y, p.z + 1, p.w)) & vpt : 0ull;
+            s_p0[tid] = ((uint64_t)p.z * g.ny + p.y) * g.nx + 64 * (uint64_t)p.w;
+            s_xyz[tid] = (uint32_t)(64 * p.w) | ((uint32_t)p.y << 16);
+            s_z[tid] = (uint32_t)p.z;
+        }
+        const uint32_t vc = popc64(ex) + popc64(ey) + popc64(ez);
+        const uint32_t incl = wave_scan_incl(vc);
+        if (lane == 63) s_wsum[wave] = incl;
+        __syncthreads();
+        uint32_t wbase = 0, vtot = 0;
+        for (uint32_t i = 0; i < 4; ++i) {
+            if (i < wave) wbase += s_wsum[i];
+            vtot += s_wsum[i];
+        }
+        const uint32_t vex = wbase + incl - vc;  // block-local first vertex of this word
+        const uint64_t vblock = a.bsum[2 * (uint64_t)gb];
+        const SegResult &r = a.res[si];
+        if (k < g.words) a.base[(uint64_t)gb * kBlock + tid] = (uint32_t)(vblock - r.v_off + vex);
+        if (r.ovf) return;  // vertex numbers do not fit 32 bits: nothing is written
+        const float *field = a.field + s.field_off;
+        const uint64_t plane = (uint64_t)g.nx * g.ny;
+        for (uint32_t r0 = 0; r0 < vtot; r0 += kVertCap) {
+            // this lane's edges, canonical order, that fall into the round's window
+            if (vc && vex < r0 + kVertCap && vex + vc > r0) {
+                uint64_t pts = ex | ey | ez;
+                uint32_t i = vex;
+                while (pts) {
+                    const uint32_t b = (uint32_t)__builtin_ctzll(pts);
+                    pts &= pts - 1;
+                    const uint32_t m =
+                        bit_at(ex, b) | (bit_at(ey, b) << 1) | (bit_at(ez, b) << 2);
 #pragma unroll
-                for (uint32_t ax = 0; ax < 3; ++ax) {
-                    if (!((m >> ax) & 1)) continue;
-                    if (i >= r0 && i < r0 + kVertCap)
-                        s_list[i - r0] = (uint16_t)((tid << 8) | (b << 2) | ax);
-                    ++i;
+                    for (uint32_t ax = 0; ax < 3; ++ax) {
+                        if (!((m >> ax) & 1)) continue;
+                        if (i >= r0 && i < r0 + kVertCap)
+                            s_list[i - r0] = (uint16_t)((tid << 8) | (b << 2) | ax);
+                        ++i;
+                    }
                 }
             }
-        }
-        __syncthreads();
-        const uint32_t n = min((uint32_t)kVertCap, vtot - r0);
-        for (uint32_t e = tid; e < n; e += kBlock) {
-            const uint32_t ent = s_list[e];
-            const uint32_t j = ent >> 8, b = (ent >> 2) & 63, ax = ent & 3;
-            const uint64_t pidx = s_p0[j] + b;
-            const uint32_t xy = s_xyz[j];
-            const float xf = (float)((xy & 0xffff) + b);
-            const float yf = (float)(xy >> 16), zf = (float)s_z[j];
-            const uint64_t step = ax == 0 ? 1ull : (ax == 1 ? (uint64_t)g.nx : plane);
-            const float f0 = a.field[pidx];
-            const float f1 = a.field[pidx + step];
-            const float t = (a.iso - f0) / (f1 - f0);
-            float px = __builtin_fmaf(xf, a.sx, a.ox);
-            float py = __builtin_fmaf(yf, a.sy, a.oy);
-            float pz = __builtin_fmaf(zf, a.sz, a.oz);
-            if (ax == 0) {
-                const float p1 = __builtin_fmaf(xf + 1.0f, a.sx, a.ox);
-                px = __builtin_fmaf(t, p1 - px, px);
-            } else if (ax == 1) {
-                const float p1 = __builtin_fmaf(yf + 1.0f, a.sy, a.oy);
-                py = __builtin_fmaf(t, p1 - py, py);
-            } else {
-                const float p1 = __builtin_fmaf(zf + 1.0f, a.sz, a.oz);
-                pz = __builtin_fmaf(t, p1 - pz, pz);
+            __syncthreads();
+            const uint32_t n = min((uint32_t)kVertCap, vtot - r0);
+            for (uint32_t e = tid; e < n; e += kBlock) {
+                const uint32_t ent = s_list[e];
+                const uint32_t j = ent >> 8, b = (ent >> 2) & 63, ax = ent & 3;
+                const uint64_t pidx = s_p0[j] + b;
+                const uint32_t xy = s_xyz[j];
+                const float xf = (float)((xy & 0xffff) + b);
+                const float yf = (float)(xy >> 16), zf = (float)s_z[j];
+                const uint64_t step = ax == 0 ? 1ull : (ax == 1 ? (uint64_t)g.nx : plane);
+                const float f0 = field[pidx];
+                const float f1 = field[pidx + step];
+                const float t = (s.iso - f0) / (f1 - f0);
+                float px = __builtin_fmaf(xf, s.sx, s.ox);
+                float py = __builtin_fmaf(yf, s.sy, s.oy);
+                float pz = __builtin_fmaf(zf, s.sz, s.oz);
+                if (ax == 0) {
+                    const float p1 = __builtin_fmaf(xf + 1.0f, s.sx, s.ox);
+                    px = __builtin_fmaf(t, p1 - px, px);
+                } else if (ax == 1) {
+                    const float p1 = __builtin_fmaf(yf + 1.0f, s.sy, s.oy);
+                    py = __builtin_fmaf(t, p1 - py, py);
+                } else {
+                    const float p1 = __builtin_fmaf(zf + 1.0f, s.sz, s.oz);
+                    pz = __builtin_fmaf(t, p1 - pz, pz);
+                }
+                const uint64_t id = vblock + r0 + e;
+                if (id < a.vcap) *(V3 *)(a.vertices + 3 * id) = V3{px, py, pz};
             }
-            const uint64_t id = vblock + r0 + e;
-            if (id < a.vcap) *(V3 *)(a.vertices + 3 * id) = V3{px, py, pz};
+            __syncthreads();
         }
-        __syncthreads();
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------
 // triangles
 // ---------------------------------------------------------------------------
 struct TriArgs {
-    const uint64_t *signs;
+    const uint64_t *signs;  // sign buffer base
     const uint32_t *base;
     const uint32_t *twave;
     const uint64_t *bsum;
     const uint64_t *tri_packed;
-    const uint64_t *ctl;
+    const SegResult *res;
     uint32_t *triangles;
     uint64_t tcap;
-    Grid g;
 };
 
-__global__ void __launch_bounds__(kBlock) tri_kernel(TriArgs a) {
+template <class Src>
+__global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q) {
     __shared__ uint64_t s_tri[256];
     __shared__ uint16_t s_list[4][kListCap];
     __shared__ uint32_t s_vid[4][12][64];
     s_tri[threadIdx.x] = a.tri_packed[threadIdx.x];
     __syncthreads();
-    if (a.ctl[CTL_OVF]) return;  // vertex ids do not fit 32 bits
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const Grid &g = a.g;
-    const uint64_t k = (uint64_t)blockIdx.x * kBlock + tid;
-    // ---- per word: sign words, crossing words, bases, active cells
-    uint64_t wa = 0, sa = 0, wb = 0, sb = 0, wc = 0, sc = 0, wd = 0, sd = 0, act = 0;
-    Cross c0 = {0, 0, 0, 0, 0}, c1 = c0, c2 = c0, c3 = c0;
-    uint32_t vb0 = 0, vb1 = 0, vb2 = 0, vb3 = 0;
-    if (k < g.words) {
-        const WordPos p = word_pos(k, g);
-        if (p.y + 1 < g.ny && p.z + 1 < g.nz) {
-            const bool has_y2 = p.y + 2 < g.ny, has_z2 = p.z + 2 < g.nz;
-            const int y = p.y, z = p.z, w = p.w;
-            wa = sword(a.signs, g, y, z, w);
-            const uint64_t na = sword(a.signs, g, y, z, w + 1);
-            wb = sword(a.signs, g, y + 1, z, w);
-            const uint64_t nb = sword(a.signs, g, y + 1, z, w + 1);
-            wc = sword(a.signs, g, y, z + 1, w);
-            const uint64_t nc = sword(a.signs, g, y, z + 1, w + 1);
-            wd = sword(a.signs, g, y + 1, z + 1, w);
-            const uint64_t nd = sword(a.signs, g, y + 1, z + 1, w + 1);
-            sa = shift1(wa, na);
-            sb = shift1(wb, nb);
-            sc = shift1(wc, nc);
-            sd = shift1(wd, nd);
-            const uint64_t ved = mask_below((int64_t)g.nx - 1 - 64 * w);
-            act = active_cells(wa, sa, wb, sb, wc, sc, wd, sd, ved);
-            if (act) {
-                const uint64_t vpt = mask_below((int64_t)g.nx - 64 * w);
-                const bool vpt_n = 64 * (w + 1) < g.nx, ved_n = 64 * (w + 1) < g.nx - 1;
-                const uint64_t we = sword(a.signs, g, y + 2, z, w),
-                               ne = sword(a.signs, g, y + 2, z, w + 1);
-                const uint64_t wf = sword(a.signs, g, y + 2, z + 1, w);
-                const uint64_t wg = sword(a.signs, g, y, z + 2, w);
-                const uint64_t wh = sword(a.signs, g, y + 1, z + 2, w);
-                c0 = row_cross(wa, na, wb, nb, wc, true, true, ved, vpt, ved_n, vpt_n);
-                c1 = row_cross(wb, nb, we, ne, wd, has_y2, true, ved, vpt, ved_n, vpt_n);
-                c2 = row_cross(wc, nc, wd, nd, wg, true, has_z2, ved, vpt, ved_n, vpt_n);
-                c3 = row_cross(wd, nd, wf, 0ull, wh, has_y2, has_z2, ved, vpt, ved_n, vpt_n);
-                const uint64_t rW = (uint64_t)g.W, rowW = (uint64_t)g.ny * g.W;
-                vb0 = a.base[k];
-                vb1 = a.base[k + rW];
-                vb2 = a.base[k + rowW];
-                vb3 = a.base[k + rowW + rW];
-            }
-        }
-    }
-    const uint32_t nact = popc64(act);
-    const uint32_t aincl = wave_scan_incl(nact);
-    const uint32_t atot = readlane32(aincl, 63);
-    const uint32_t aex = aincl - nact;
-    uint16_t *list = s_list[wave];
-    uint64_t tnext = a.bsum[2 * blockIdx.x + 1];
-    for (uint32_t i = 0; i < wave; ++i) tnext += a.twave[4 * (uint64_t)blockIdx.x + i];
-    uint32_t (*vid)[64] = s_vid[wave];
-    for (uint32_t r0 = 0; r0 < atot; r0 += kListCap) {
-        fill_list(list, act, nact, aex, r0, lane);
-        const uint32_t n = min((uint32_t)kListCap, atot - r0);
-        for (uint32_t e0 = 0; e0 < n; e0 += 64) {
-            const uint32_t e = e0 + lane;
-            const bool on = e < n;
-            const uint32_t ent = on ? list[e] : 0u;
-            const uint32_t j = ent & 63, x = ent >> 6;
-            const uint32_t cube = cube_at(gather64(wa, j), gather64(sa, j), gather64(wb, j),
-                                          gather64(sb, j), gather64(wc, j), gather64(sc, j),
-                                          gather64(wd, j), gather64(sd, j), x);
-            const uint64_t packed = s_tri[cube];
-            const uint32_t nt = on ? (uint32_t)(packed >> 60) : 0u;
-            const uint32_t tincl = wave_scan_incl(nt);
-            const uint64_t t0 = tnext + (tincl - nt);
-            tnext += readlane32(tincl, 63);
-            const uint64_t below = x ? (~0ull >> (64 - x)) : 0ull;
-            const uint64_t ex0 = gather64(c0.ex, j), ey0 = gather64(c0.ey, j), ez0 = gather64(c0.ez, j);
-            const uint64_t ex1 = gather64(c1.ex, j), ey1 = gather64(c1.ey, j), ez1 = gather64(c1.ez, j);
-            const uint64_t ex2 = gather64(c2.ex, j), ey2 = gather64(c2.ey, j), ez2 = gather64(c2.ez, j);
-            const uint64_t ex3 = gather64(c3.ex, j), ey3 = gather64(c3.ey, j), ez3 = gather64(c3.ez, j);
-            const uint32_t sx0 = bit_at(gather64(c0.sx, j), x), sy0 = bit_at(gather64(c0.sy, j), x);
-            const uint32_t sx1 = bit_at(gather64(c1.sx, j), x), sy1 = bit_at(gather64(c1.sy, j), x);
-            const uint32_t sx2 = bit_at(gather64(c2.sx, j), x);
-            const uint32_t p0 = gather32(vb0, j) + popc64(ex0 & below) + popc64(ey0 & below) + popc64(ez0 & below);
-            const uint32_t p1 = gather32(vb1, j) + popc64(ex1 & below) + popc64(ey1 & below) + popc64(ez1 & below);
-            const uint32_t p2 = gather32(vb2, j) + popc64(ex2 & below) + popc64(ey2 & below) + popc64(ez2 & below);
-            const uint32_t p3 = gather32(vb3, j) + popc64(ex3 & below) + popc64(ey3 & below) + popc64(ez3 & below);
-            const uint32_t x0 = bit_at(ex0, x), y0 = bit_at(ey0, x), z0 = bit_at(ez0, x);
-            const uint32_t x1 = bit_at(ex1, x), y1 = bit_at(ey1, x), z1 = bit_at(ez1, x);
-            const uint32_t x2 = bit_at(ex2, x), y2 = bit_at(ey2, x), z2 = bit_at(ez2, x);
-            const uint32_t q0 = p0 + x0 + y0 + z0;  // first edge of point x+1
-            const uint32_t q1 = p1 + x1 + y1 + z1;
-            const uint32_t q2 = p2 + x2 + y2 + z2;
-            if (on) {
-                vid[0][lane] = p0;                 // x-edge of (x,   y,   z)
-                vid[1][lane] = q0 + sx0;           // y-edge of (x+1, y,   z)
-                vid[2][lane] = p1;                 // x-edge of (x,   y+1, z)
-                vid[3][lane] = p0 + x0;            // y-edge of (x,   y,   z)
-                vid[4][lane] = p2;                 // x-edge of (x,   y,   z+1)
-                vid[5][lane] = q2 + sx2;           // y-edge of (x+1, y,   z+1)
-                vid[6][lane] = p3;                 // x-edge of (x,   y+1, z+1)
-                vid[7][lane] = p2 + x2;            // y-edge of (x,   y,   z+1)
-                vid[8][lane] = p0 + x0 + y0;       // z-edge of (x,   y,   z)
-                vid[9][lane] = q0 + sx0 + sy0;     // z-edge of (x+1, y,   z)
-                vid[10][lane] = q1 + sx1 + sy1;    // z-edge of (x+1, y+1, z)
-                vid[11][lane] = p1 + x1 + y1;      // z-edge of (x,   y+1, z)
-                for (uint32_t jt = 0; jt < nt; ++jt) {
-                    const uint64_t t = t0 + jt;
-                    if (t < a.tcap) {
-                        const uint32_t v0 = vid[(packed >> (12 * jt)) & 15][lane];
-                        const uint32_t v1 = vid[(packed >> (12 * jt + 4)) & 15][lane];
-                        const uint32_t v2 = vid[(packed >> (12 * jt + 8)) & 15][lane];
-                        *(T3 *)(a.triangles + 3 * t) = T3{v0, v1, v2};
-                    }
+    for_each_block(q, [&](uint32_t gb) {
+        const uint32_t si = src.seg_of(gb);
+        if (a.res[si].ovf) return;  // vertex numbers do not fit 32 bits
+        const Seg s = src.get(si);
+        const Grid &g = s.g;
+        const uint64_t *signs = a.signs + s.sign_off;
+        const uint64_t k = (uint64_t)(gb - s.block_off) * kBlock + tid;
+        const uint32_t *base = a.base + (uint64_t)s.block_off * kBlock;  // by grid word
+        // ---- per word: sign words, crossing words, bases, active cells
+        uint64_t wa = 0, sa = 0, wb = 0, sb = 0, wc = 0, sc = 0, wd = 0, sd = 0, act = 0;
+        Cross c0 = {0, 0, 0, 0, 0}, c1 = c0, c2 = c0, c3 = c0;
+        uint32_t vb0 = 0, vb1 = 0, vb2 = 0, vb3 = 0;
+        if (k < g.words) {
+            const WordPos p = word_pos(k, g);
+            if (p.y + 1 < g.ny && p.z + 1 < g.nz) {
+                const bool has_y2 = p.y + 2 < g.ny, has_z2 = p.z + 2 < g.nz;
+                const int y = p.y, z = p.z, w = p.w;
+                wa = sword(signs, g, y, z, w);
+                const uint64_t na = sword(signs, g, y, z, w + 1);
+                wb = sword(signs, g, y + 1, z, w);
+                const uint64_t nb = sword(signs, g, y + 1, z, w + 1);
+                wc = sword(signs, g, y, z + 1, w);
+                const uint64_t nc = sword(signs, g, y, z + 1, w + 1);
+                wd = sword(signs, g, y + 1, z + 1, w);
+                const uint64_t nd = sword(signs, g, y + 1, z + 1, w + 1);
+                sa = shift1(wa, na);
+                sb = shift1(wb, nb);
+                sc = shift1(wc, nc);
+                sd = shift1(wd, nd);
+                const uint64_t ved = mask_below((int64_t)g.nx - 1 - 64 * w);
+                act = active_cells(wa, sa, wb, sb, wc, sc, wd, sd, ved);
+                if (act) {
+                    const uint64_t vpt = mask_below((int64_t)g.nx - 64 * w);
+                    const bool vpt_n = 64 * (w + 1) < g.nx, ved_n = 64 * (w + 1) < g.nx - 1;
+                    const uint64_t we = sword(signs, g, y + 2, z, w),
+                                   ne = sword(signs, g, y + 2, z, w + 1);
+                    const uint64_t wf = sword(signs, g, y + 2, z + 1, w);
+                    const uint64_t wg = sword(signs, g, y, z + 2, w);
+                    const uint64_t wh = sword(signs, g, y + 1, z + 2, w);
+                    c0 = row_cross(wa, na, wb, nb, wc, true, true, ved, vpt, ved_n, vpt_n);
+                    c1 = row_cross(wb, nb, we, ne, wd, has_y2, true, ved, vpt, ved_n, vpt_n);
+                    c2 = row_cross(wc, nc, wd, nd, wg, true, has_z2, ved, vpt, ved_n, vpt_n);
+                    c3 = row_cross(wd, nd, wf, 0ull, wh, has_y2, has_z2, ved, vpt, ved_n, vpt_n);
+                    const uint64_t rW = (uint64_t)g.W, rowW = (uint64_t)g.ny * g.W;
+                    vb0 = base[k];
+                    vb1 = base[k + rW];
+                    vb2 = base[k + rowW];
+                    vb3 = base[k + rowW + rW];
                 }
             }
         }
-        __builtin_amdgcn_wave_barrier();
-    }
+        const uint32_t nact = popc64(act);
+        const uint32_t aincl = wave_scan_incl(nact);
+        const uint32_t atot = readlane32(aincl, 63);
+        const uint32_t aex = aincl - nact;
+        uint16_t *list = s_list[wave];
+        uint64_t tnext = a.bsum[2 * (uint64_t)gb + 1];
+        for (uint32_t i = 0; i < wave; ++i) tnext += a.twave[4 * (uint64_t)gb + i];
+        uint32_t(*vid)[64] = s_vid[wave];
+        for (uint32_t r0 = 0; r0 < atot; r0 += kListCap) {
+            fill_list(list, act, nact, aex, r0, lane);
+            const uint32_t n = min((uint32_t)kListCap, atot - r0);
+            for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+                const uint32_t e = e0 + lane;
+                const bool on = e < n;
+                const uint32_t ent = on ? list[e] : 0u;
+                const uint32_t j = ent & 63, x = ent >> 6;
+                const uint32_t cube = cube_at(gather64(wa, j), gather64(sa, j), gather64(wb, j),
+                                              gather64(sb, j), gather64(wc, j), gather64(sc, j),
+                                              gather64(wd, j), gather64(sd, j), x);
+                const uint64_t packed = s_tri[cube];
+                const uint32_t nt = on ? (uint32_t)(packed >> 60) : 0u;
+                const uint32_t tincl = wave_scan_incl(nt);
+                const uint64_t t0 = tnext + (tincl - nt);
+                tnext += readlane32(tincl, 63);
+                const uint64_t below = x ? (~0ull >> (64 - x)) : 0ull;
+                const uint64_t ex0 = gather64(c0.ex, j), ey0 = gather64(c0.ey, j),
+                               ez0 = gather64(c0.ez, j);
+                const uint64_t ex1 = gather64(c1.ex, j), ey1 = gather64(c1.ey, j),
+                               ez1 = gather64(c1.ez, j);
+                const uint64_t ex2 = gather64(c2.ex, j), ey2 = gather64(c2.ey, j),
+                               ez2 = gather64(c2.ez, j);
+                const uint64_t ex3 = gather64(c3.ex, j), ey3 = gather64(c3.ey, j),
+                               ez3 = gather64(c3.ez, j);
+                const uint32_t sx0 = bit_at(gather64(c0.sx, j), x),
+                               sy0 = bit_at(gather64(c0.sy, j), x);
+                const uint32_t sx1 = bit_at(gather64(c1.sx, j), x),
+                               sy1 = bit_at(gather64(c1.sy, j), x);
+                const uint32_t sx2 = bit_at(gather64(c2.sx, j), x);
+                const uint32_t p0 = gather32(vb0, j) + popc64(ex0 & below) +
+                                    popc64(ey0 & below) + popc64(ez0 & below);
+                const uint32_t p1 = gather32(vb1, j) + popc64(ex1 & below) +
+                                    popc64(ey1 & below) + popc64(ez1 & below);
+                const uint32_t p2 = gather32(vb2, j) + popc64(ex2 & below) +
+                                    popc64(ey2 & below) + popc64(ez2 & below);
+                const uint32_t p3 = gather32(vb3, j) + popc64(ex3 & below) +
+                                    popc64(ey3 & below) + popc64(ez3 & below);
+                const uint32_t x0 = bit_at(ex0, x), y0 = bit_at(ey0, x), z0 = bit_at(ez0, x);
+                const uint32_t x1 = bit_at(ex1, x), y1 = bit_at(ey1, x), z1 = bit_at(ez1, x);
+                const uint32_t x2 = bit_at(ex2, x), y2 = bit_at(ey2, x), z2 = bit_at(ez2, x);
+                const uint32_t q0 = p0 + x0 + y0 + z0;  // first edge of point x+1
+                const uint32_t q1 = p1 + x1 + y1 + z1;
+                const uint32_t q2 = p2 + x2 + y2 + z2;
+                if (on) {
+                    vid[0][lane] = p0;              // x-edge of (x,   y,   z)
+                    vid[1][lane] = q0 + sx0;        // y-edge of (x+1, y,   z)
+                    vid[2][lane] = p1;              // x-edge of (x,   y+1, z)
+                    vid[3][lane] = p0 + x0;         // y-edge of (x,   y,   z)
+                    vid[4][lane] = p2;              // x-edge of (x,   y,   z+1)
+                    vid[5][lane] = q2 + sx2;        // y-edge of (x+1, y,   z+1)
+                    vid[6][lane] = p3;              // x-edge of (x,   y+1, z+1)
+                    vid[7][lane] = p2 + x2;         // y-edge of (x,   y,   z+1)
+                    vid[8][lane] = p0 + x0 + y0;    // z-edge of (x,   y,   z)
+                    vid[9][lane] = q0 + sx0 + sy0;  // z-edge of (x+1, y,   z)
+                    vid[10][lane] = q1 + sx1 + sy1; // z-edge of (x+1, y+1, z)
+                    vid[11][lane] = p1 + x1 + y1;   // z-edge of (x,   y+1, z)
+                    for (uint32_t jt = 0; jt < nt; ++jt) {
+                        const uint64_t t = t0 + jt;
+                        if (t < a.tcap) {
+                            const uint32_t v0 = vid[(packed >> (12 * jt)) & 15][lane];
+                            const uint32_t v1 = vid[(packed >> (12 * jt + 4)) & 15][lane];
+                            const uint32_t v2 = vid[(packed >> (12 * jt + 8)) & 15][lane];
+                            *(T3 *)(a.triangles + 3 * t) = T3{v0, v1, v2};
+                        }
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    });
 }
 
 }  // namespace
 
-// Enqueues the whole extraction on `stream` given a workspace (also used by
-// the batched and served forms).
+// Enqueues the whole one-grid extraction on `stream` given a workspace.
 int enqueue_extract(const gc_extract_args *a, hipStream_t stream) {
     GC_ARG(a, "args is NULL");
     GC_ARG(a->field, "field is NULL");
@@ -628,74 +711,96 @@ int enqueue_extract(const gc_extract_args *a, hipStream_t stream) {
     int st = device_tables(&tabs);
     if (st) return st;
     char *ws = (char *)a->workspace;
-    Grid g;
-    g.nx = a->nx;
-    g.ny = a->ny;
-    g.nz = a->nz;
-    g.W = (int)gpucad_sign_words(a->nx, 1, 1);
-    g.words = L.words;
-    g.divW = make_div((uint32_t)g.W);
-    g.divNy = make_div((uint32_t)g.ny);
+    OneGrid one;
+    Seg &s = one.s;
+    std::memset(&s, 0, sizeof(s));
+    s.g = make_grid(a->nx, a->ny, a->nz);
+    s.iso = a->iso;
+    s.ox = a->origin.x;
+    s.oy = a->origin.y;
+    s.oz = a->origin.z;
+    s.sx = a->spacing.x;
+    s.sy = a->spacing.y;
+    s.sz = a->spacing.z;
     const uint64_t *signs = a->signs;
     if (!signs) {
         GC_ARG((a->nz + kSignZ - 1) / kSignZ <= 65535 && (a->ny + 3) / 4 <= 65535,
                "grid too large");
-        uint64_t *s = (uint64_t *)(ws + L.signs_off);
-        dim3 grid(g.W, (a->ny + 3) / 4, (a->nz + kSignZ - 1) / kSignZ);
-        hipLaunchKernelGGL(signs_kernel, grid, dim3(256), 0, stream, a->field, s, a->nx, a->ny,
-                           a->nz, g.W, a->iso);
+        uint64_t *sw = (uint64_t *)(ws + L.signs_off);
+        dim3 grid(s.g.W, (a->ny + 3) / 4, (a->nz + kSignZ - 1) / kSignZ);
+        hipLaunchKernelGGL(signs_kernel, grid, dim3(256), 0, stream, a->field, sw, a->nx, a->ny,
+                           a->nz, s.g.W, a->iso);
         GC_CHECK_LAUNCH();
-        signs = s;
+        signs = sw;
     }
-    uint32_t *vcnt = (uint32_t *)(ws + L.vcnt_off);
     uint32_t *base = (uint32_t *)(ws + L.base_off);
     uint32_t *twave = (uint32_t *)(ws + L.twave_off);
     uint64_t *bsum = (uint64_t *)(ws + L.bsum_off);
     uint64_t *ctl = (uint64_t *)(ws + L.ctl_off);
+    SegResult *res = (SegResult *)(ws + L.res_off);
     const dim3 blocks((unsigned)L.nb);
-    hipLaunchKernelGGL(count_kernel, blocks, dim3(kBlock), 0, stream, signs, vcnt, twave, bsum,
-                       tabs->num_verts, g);
+    const Queue none = {nullptr, 0};
+    hipLaunchKernelGGL(count_kernel<OneGrid>, blocks, dim3(kBlock), 0, stream, one,
+                       CountArgs{signs, twave, bsum, tabs->num_verts}, none);
     GC_CHECK_LAUNCH();
     hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, stream, bsum, L.nb, ctl,
-                       a->d_counts);
+                       a->d_counts, (const Seg *)nullptr, 1u, res);
     GC_CHECK_LAUNCH();
-
-    VertArgs va;
-    va.field = a->field;
-    va.signs = signs;
-    va.vcnt = vcnt;
-    va.bsum = bsum;
-    va.ctl = ctl;
-    va.base = base;
-    va.vertices = a->vertices;
-    va.vcap = a->vertices ? a->vert_capacity : 0;
-    va.g = g;
-    va.iso = a->iso;
-    va.ox = a->origin.x;
-    va.oy = a->origin.y;
-    va.oz = a->origin.z;
-    va.sx = a->spacing.x;
-    va.sy = a->spacing.y;
-    va.sz = a->spacing.z;
-    hipLaunchKernelGGL(vert_kernel, blocks, dim3(kBlock), 0, stream, va);
+    const VertArgs va = {a->field, signs, bsum, res, base, a->vertices,
+                         a->vertices ? a->vert_capacity : 0};
+    hipLaunchKernelGGL(vert_kernel<OneGrid>, blocks, dim3(kBlock), 0, stream, one, va, none);
     GC_CHECK_LAUNCH();
-
-    TriArgs ta;
-    ta.signs = signs;
-    ta.base = base;
-    ta.twave = twave;
-    ta.bsum = bsum;
-    ta.tri_packed = tabs->tri_packed;
-    ta.ctl = ctl;
-    ta.triangles = a->triangles;
-    ta.tcap = a->triangles ? a->tri_capacity : 0;
-    ta.g = g;
-    hipLaunchKernelGGL(tri_kernel, blocks, dim3(kBlock), 0, stream, ta);
+    const TriArgs ta = {signs, base, twave, bsum, tabs->tri_packed, res, a->triangles,
+                        a->triangles ? a->tri_capacity : 0};
+    hipLaunchKernelGGL(tri_kernel<OneGrid>, blocks, dim3(kBlock), 0, stream, one, ta, none);
     GC_CHECK_LAUNCH();
     return GPUCAD_OK;
 }
 
-uint64_t extract_ctl_offset(int nx, int ny, int nz) { return layout(nx, ny, nz).ctl_off; }
+// Workgroups of each many-grid stage that fit on one CU at once.
+int extract_batch_occupancy(int *count, int *vert, int *tri) {
+    GC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(count, count_kernel<ManyGrids>, kBlock, 0));
+    GC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(vert, vert_kernel<ManyGrids>, kBlock, 0));
+    GC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(tri, tri_kernel<ManyGrids>, kBlock, 0));
+    *count = std::max(1, *count);
+    *vert = std::max(1, *vert);
+    *tri = std::max(1, *tri);
+    return GPUCAD_OK;
+}
+
+int enqueue_extract_batch(const BatchExtract &b, int phases, hipStream_t stream) {
+    GC_ARG(b.nseg > 0 && b.nb > 0 && b.segs && b.blk2seg && b.queues, "empty batch");
+    const DeviceTables *tabs = nullptr;
+    int st = device_tables(&tabs);
+    if (st) return st;
+    const ManyGrids src = {b.segs, b.blk2seg};
+    const uint32_t wmax = b.nb;
+    if (phases & EXTRACT_COUNT) {
+        const unsigned wg = (unsigned)std::max(1, std::min<int>(b.wg_count, (int)wmax));
+        hipLaunchKernelGGL(count_kernel<ManyGrids>, dim3(wg), dim3(kBlock), 0, stream, src,
+                           CountArgs{b.signs, b.twave, b.bsum, tabs->num_verts},
+                           Queue{b.queues + 0, b.nb});
+        GC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, stream, b.bsum,
+                           (uint64_t)b.nb, b.ctl, (uint64_t *)nullptr, b.segs, b.nseg, b.res);
+        GC_CHECK_LAUNCH();
+    }
+    if (phases & EXTRACT_EMIT) {
+        const unsigned wv = (unsigned)std::max(1, std::min<int>(b.wg_vert, (int)wmax));
+        const VertArgs va = {b.fields, b.signs, b.bsum, b.res, b.base, b.vertices,
+                             b.vertices ? b.vcap : 0};
+        hipLaunchKernelGGL(vert_kernel<ManyGrids>, dim3(wv), dim3(kBlock), 0, stream, src, va,
+                           Queue{b.queues + 1, b.nb});
+        GC_CHECK_LAUNCH();
+        const unsigned wt = (unsigned)std::max(1, std::min<int>(b.wg_tri, (int)wmax));
+        const TriArgs ta = {b.signs, b.base, b.twave, b.bsum, tabs->tri_packed, b.res,
+                            b.triangles, b.triangles ? b.tcap : 0};
+        hipLaunchKernelGGL(tri_kernel<ManyGrids>, dim3(wt), dim3(kBlock), 0, stream, src, ta,
+                           Queue{b.queues + 2, b.nb});
+        GC_CHECK_LAUNCH();
+    }
+    return GPUCAD_OK;
+}
 
 }  // namespace gc
 

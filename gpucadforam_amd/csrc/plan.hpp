@@ -1,0 +1,61 @@
+// Launch plans of the many-grid forms (batched: batch.hip, served: server.cpp):
+// which grids, where their fields and sign words live, the brick queue of the
+// field launches and the block -> grid table of the extraction launches.
+#pragma once
+
+#include <vector>
+
+#include "extract_types.hpp"
+#include "lattice_body.hpp"
+
+namespace gc {
+
+constexpr int kVariants = 12;  // field variants: rotation x tpms x thickness modes
+constexpr int kQueues = 16;    // queue counters: 0..2 extraction stages, 3.. field variants
+
+inline uint64_t round_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+
+struct Shape {
+    int bx = 64, lz = 16, wpc = 8;   // field brick (x points per wave row, z planes) and
+                                     // resident waves per CU of the field launches
+    int xc = 2, xv = 2, xt = 2;      // extraction workgroups per CU: count, vert, tri
+};
+
+struct HostPlan {
+    std::vector<FieldConsts> consts;
+    std::vector<Seg> segs;
+    std::vector<uint32_t> blk2seg;
+    std::vector<uint64_t> items;     // seg | brick x << 16 | brick y << 32 | brick z << 48
+    uint32_t vbegin[kVariants + 1] = {};
+    uint32_t nb = 0;
+};
+
+// Device buffers of one plan.
+struct DevPlan {
+    FieldConsts *consts = nullptr;
+    uint64_t *items = nullptr;
+    Seg *segs = nullptr;
+    uint32_t *blk2seg = nullptr;
+    SegResult *res = nullptr;
+    uint32_t *base = nullptr, *twave = nullptr;
+    uint64_t *bsum = nullptr, *ctl = nullptr;
+    uint32_t *queues = nullptr;
+};
+
+int validate_batch_grid(const gc_lattice_params &p, int i);
+uint64_t brick_count(const gc_lattice_params &p, int bx, int lz);
+void plan_segments(const gc_lattice_params *p, int n, const uint64_t *field_off,
+                   const uint64_t *sign_off, float iso, int unit_domain, HostPlan &h);
+void plan_items(const gc_lattice_params *p, int n, const Shape &sh, HostPlan &h);
+int enqueue_field_plan(const HostPlan &h, const DevPlan &d, const Shape &sh, int cus,
+                       float *fields, uint64_t *signs, hipStream_t stream);
+BatchExtract make_batch_extract(const HostPlan &h, const DevPlan &d, const Shape &sh, int cus,
+                                const float *fields, const uint64_t *signs, float *vertices,
+                                uint64_t vcap, uint32_t *triangles, uint64_t tcap);
+int zero_queues(const DevPlan &d, hipStream_t stream);
+int device_cus(int *cus);
+int extract_batch_occupancy(int *count, int *vert, int *tri);
+// Default shape: brick 64 x 4 x 16, 8 waves per CU, extraction at full occupancy.
+int default_shape(Shape *s);
+
+}  // namespace gc

@@ -153,8 +153,8 @@ typedef struct gc_extract_args {
     uint64_t vert_capacity;
     uint32_t *triangles;       /* device uint32[3*tri_capacity] */
     uint64_t tri_capacity;
-    float *normals;            /* device float3[vert_capacity] or NULL: per-vertex
-                                  normalised central-difference gradient */
+    float *normals;            /* reserved for per-vertex normals: must be NULL
+                                  (GPUCAD_E_ARG otherwise) */
     void *workspace;           /* device, >= gpucad_extract_workspace_bytes() */
     uint64_t workspace_bytes;
     uint64_t *d_counts;        /* device uint64[2] = {vertices, triangles}, or NULL */
@@ -167,6 +167,137 @@ uint64_t gpucad_extract_workspace_bytes(int32_t nx, int32_t ny, int32_t nz);
  * completion, fills counts and returns GPUCAD_E_OVERFLOW when a capacity was
  * too small (nothing is written past a capacity in any case). */
 int gpucad_extract(const gc_extract_args *a, gc_mesh_counts *counts, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched form: many lattice grids of uneven sizes and types per launch    */
+/* (no counterpart in the reference)                                        */
+/* ------------------------------------------------------------------------ */
+/* A batch owns the fields, sign words, extraction workspace and mesh buffers
+ * of its grids.  gpucad_batch_run evaluates every field and extracts every
+ * mesh with persistent workgroups fed by device queues of bricks (field) and
+ * word blocks (extraction); with use_graph the whole run is one replay of a
+ * captured HIP graph.  Each grid's mesh is bit-identical to what
+ * gpucad_lattice_field + gpucad_extract produce for it alone. */
+typedef struct gc_batch gc_batch;
+
+typedef struct gc_batch_options {
+    float iso;                 /* isovalue of every grid */
+    int32_t brick_x;           /* points along x per wave row: 64, 32, 16, 8; 0 = autotuned */
+    int32_t brick_lz;          /* z planes per brick, 1..128; 0 = autotuned */
+    int32_t waves_per_cu;      /* resident waves per CU of the persistent field
+                                  launch, 4..32 in steps of 4; 0 = autotuned */
+    int32_t use_graph;         /* nonzero: a run is one replay of a captured graph */
+    int32_t unit_domain;       /* vertex positions: 0 = point indices, 1 = [0,1]^3
+                                  (spacing 1/(n-1) per axis) */
+    uint64_t vert_capacity;    /* total over the batch; 0 = sized at creation */
+    uint64_t tri_capacity;     /* (by a counting run of the batch) */
+} gc_batch_options;
+
+typedef struct gc_batch_grid_result {
+    uint64_t field_offset;     /* first point of the grid's field (floats) */
+    uint64_t vert_offset;      /* the grid's vertices: [vert_offset, +n_vertices) */
+    uint64_t n_vertices;
+    uint64_t tri_offset;       /* its triangles; corners are vertex numbers */
+    uint64_t n_triangles;      /* relative to vert_offset */
+    int32_t status;            /* GPUCAD_OK or GPUCAD_E_OVERFLOW (mesh not complete) */
+    int32_t pad;
+} gc_batch_grid_result;
+
+typedef struct gc_batch_buffers {
+    float *fields;             /* device, field_floats */
+    uint64_t field_floats;
+    float *vertices;           /* device float3[vert_capacity] */
+    uint64_t vert_capacity;
+    uint32_t *triangles;       /* device uint32[3*tri_capacity] */
+    uint64_t tri_capacity;
+} gc_batch_buffers;
+
+/* Validates the grids, allocates the batch, autotunes what the options leave
+ * at 0 (timing candidates on this batch; choices are cached per device and
+ * batch signature, and in the file named by GPUCAD_TUNE_CACHE if set), sizes
+ * the mesh buffers and captures the graph.  Grids: 1..65535, each with
+ * nx <= 65536 and fewer than 2^32 sign words. */
+int gpucad_batch_create(const gc_lattice_params *grids, int32_t n_grids,
+                        const gc_batch_options *opt, gc_batch **out);
+/* Enqueues one run of the whole batch on stream (asynchronous). */
+int gpucad_batch_run(gc_batch *b, void *stream);
+/* Waits for the last run and fills res[0..n) (n = the batch's grid count). */
+int gpucad_batch_results(gc_batch *b, gc_batch_grid_result *res, int32_t n);
+int gpucad_batch_buffers(const gc_batch *b, gc_batch_buffers *out);
+/* The options in use after autotuning. */
+int gpucad_batch_tuning(const gc_batch *b, gc_batch_options *chosen);
+int gpucad_batch_destroy(gc_batch *b);
+
+/* ------------------------------------------------------------------------ */
+/* Served form: lattice-edit requests as they arrive (no reference          */
+/* counterpart)                                                              */
+/* ------------------------------------------------------------------------ */
+/* A server owns one device arena of budget_bytes.  Submitted requests wait
+ * in a FIFO; each call of gpucad_server_poll advances the loop: requests are
+ * admitted into a launch group while their field, sign words and extraction
+ * work fit in the arena (up to max_batch per group), a group runs on one of
+ * `slots` streams (field + count, then, once its mesh sizes are known and
+ * mesh memory is allocated, vertices + triangles), and groups on different
+ * streams overlap on the device.  Requests that arrive while groups run are
+ * admitted as soon as a stream and memory are free (continuous batching).
+ * A finished request's field and mesh stay resident in the arena until
+ * gpucad_server_release.  Each mesh is bit-identical to what
+ * gpucad_lattice_field + gpucad_extract produce for the request alone
+ * (vertex positions in [0,1]^3: spacing 1/(n-1) per axis). */
+typedef struct gc_server gc_server;
+
+typedef struct gc_server_options {
+    uint64_t budget_bytes;     /* device arena size (fields, meshes, work) */
+    int32_t max_batch;         /* requests per launch group, 1..65535; 0 = 256 */
+    int32_t slots;             /* launch groups in flight (streams), 1..8; 0 = 3 */
+    float iso;
+    int32_t brick_x;           /* field brick (as gc_batch_options); 0 = 64 */
+    int32_t brick_lz;          /* 0 = 16 */
+    int32_t waves_per_cu;      /* 0 = 8 */
+} gc_server_options;
+
+typedef struct gc_request_result {
+    uint64_t ticket;
+    int32_t status;            /* GPUCAD_OK, or GPUCAD_E_OOM: larger than the budget */
+    int32_t pad;
+    const float *field;        /* device, nx*ny*nz floats */
+    const float *vertices;     /* device float3[n_vertices] */
+    const uint32_t *triangles; /* device uint32[3*n_triangles]; vertex numbers of this mesh */
+    uint64_t n_vertices, n_triangles;
+    double submit_s, done_s;   /* seconds on the server's clock */
+} gc_request_result;
+
+typedef struct gc_server_stats {
+    uint64_t submitted, completed, rejected, groups;
+    uint64_t bytes_in_use, peak_bytes_in_use, budget_bytes;
+    double mean_group_size;
+} gc_server_stats;
+
+int gpucad_server_create(const gc_server_options *opt, gc_server **out);
+/* Queues a request (the parameters are copied); returns its ticket. */
+int gpucad_server_submit(gc_server *s, const gc_lattice_params *p, uint64_t *ticket);
+/* Advances the loop without waiting and returns up to max finished requests
+ * (each is returned once); *n_done receives how many. */
+int gpucad_server_poll(gc_server *s, gc_request_result *done, int32_t max, int32_t *n_done);
+/* Frees a finished request's field and mesh. */
+int gpucad_server_release(gc_server *s, uint64_t ticket);
+int gpucad_server_stats(const gc_server *s, gc_server_stats *out);
+/* Waits for all work in flight and frees the arena. */
+int gpucad_server_destroy(gc_server *s);
+/* Drives the server with n requests arriving at arrival_s[i] seconds after
+ * the start (non-decreasing): submits each on time, polls continuously,
+ * releases each result when it completes, and writes latency_s[i] = done -
+ * arrival.  *wall_s receives the time from start to the last completion.
+ * Requests with status other than OK get latency -1. */
+int gpucad_server_run_trace(gc_server *s, const gc_lattice_params *reqs,
+                            const double *arrival_s, int32_t n, double *latency_s,
+                            double *wall_s);
+/* The same requests served by plain per-request calls, one at a time in
+ * arrival order (gpucad_lattice_field with sign words, then gpucad_extract
+ * with its synchronous counts, into buffers allocated once for the largest
+ * request): the baseline the served form is compared with. */
+int gpucad_plain_run_trace(const gc_lattice_params *reqs, const double *arrival_s, int32_t n,
+                           float iso, double *latency_s, double *wall_s);
 
 /* ------------------------------------------------------------------------ */
 /* Drop-in field entry points                                               */

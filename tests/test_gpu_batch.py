@@ -1,0 +1,176 @@
+"""GPU tests of the batched form (gpucad_batch_*).
+
+A grid's field, sign words and mesh inside a batch must be bit-identical to
+what the one-grid entry points (gpucad_lattice_field + gpucad_extract)
+produce for it alone, whatever the brick shape, the waves per CU, graph
+replay or not; a couple of grids are also checked against the oracle
+directly.  Graph replays are repeated with eager work in between and every
+replay is compared.
+"""
+import numpy as np
+import pytest
+import torch
+
+from gpucadforam_amd import configs as CF
+from tests.helpers import to_np
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_grids():
+    """Uneven sizes and every field variant family: plain, ramp, rotation,
+    thickness modes, bend, primitives; x sizes that are not multiples of 64
+    (nor of 8)."""
+    g = list(CF.batched_grids(6, seed=0x1234))
+    g.append(CF.make(70, 66, 40, 0, 10.0, 30.0, sheet=True, t0=0.2))
+    g.append(CF.make(72, 40, 33, 0, 12.0, rot0=0.3, rot1=1.2, sheet=True, t0=0.25,
+                     modes=CF.fourier_modes(7)))
+    g.append(CF.make(65, 64, 31, 2, 14.0, 20.0, rot0=-0.5, rot1=0.5, sheet=False, prim=2, op=3,
+                     half=(0.3, 0.35, 0.4)))
+    g.append(CF.bent_diamond(48))
+    g.append(CF.varying_schwarz_p(57))
+    g.append(CF.make(3, 130, 9, 1, 6.0, sheet=True, t0=0.3))
+    g.extend(CF.served_requests(3, seed=0x77))
+    return g
+
+
+GRIDS = _mixed_grids()
+
+
+def _alone(gc, p):
+    f = torch.empty(p.npoints(), dtype=torch.float32, device="cuda")
+    s = torch.zeros(gc.sign_words(p.nx, p.ny, p.nz), dtype=torch.int64, device="cuda")
+    gc.lattice_field(p, out=f, signs=s, iso=0.0)
+    m = gc.extract(f, p.nx, p.ny, p.nz, signs=s)
+    torch.cuda.synchronize()
+    return to_np(f), to_np(m.vertices), to_np(m.triangles)
+
+
+@pytest.fixture(scope="module")
+def alone(gc, dev):
+    return [_alone(gc, p) for p in GRIDS]
+
+
+def _check_batch(b, alone, label):
+    res = b.results()
+    for i, (f_ref, v_ref, t_ref) in enumerate(alone):
+        f = to_np(b.field(i, res))
+        assert np.array_equal(f.view(np.uint32), f_ref.view(np.uint32)), "%s grid %d field" % (
+            label, i)
+        m = b.mesh(i, res)
+        assert (m.n_vertices, m.n_triangles) == (len(v_ref), len(t_ref)), "%s grid %d counts" % (
+            label, i)
+        v = to_np(m.vertices)
+        t = to_np(m.triangles)
+        assert np.array_equal(v.view(np.uint32), v_ref.view(np.uint32)), "%s grid %d vertices" % (
+            label, i)
+        assert np.array_equal(t, t_ref), "%s grid %d triangles" % (label, i)
+    # meshes are packed back to back in grid order
+    off_v = off_t = 0
+    for r in res:
+        assert r["vert_offset"] == off_v and r["tri_offset"] == off_t
+        off_v += r["n_vertices"]
+        off_t += r["n_triangles"]
+
+
+@pytest.mark.parametrize("bx", [64, 32, 16, 8])
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_matches_alone(gc, dev, alone, bx, graph):
+    print("batch bx=%d graph=%s grids=%d" % (bx, graph, len(GRIDS)), flush=True)
+    b = gc.Batch(GRIDS, brick_x=bx, brick_lz=8 if bx < 64 else 16, waves_per_cu=8,
+                 use_graph=graph)
+    try:
+        b.run()
+        _check_batch(b, alone, "bx%d graph%d" % (bx, graph))
+    finally:
+        b.close()
+
+
+def test_graph_replays_are_identical(gc, dev, alone):
+    b = gc.Batch(GRIDS, brick_x=16, brick_lz=16, waves_per_cu=16, use_graph=True)
+    try:
+        for r in range(3):
+            # eager work between replays, on the same stream
+            junk = torch.randn(1 << 20, device="cuda")
+            junk.mul_(2.0).sum().item()
+            b.fields.fill_(float("nan"))
+            b.vertices.zero_()
+            b.triangles.fill_(-1)
+            torch.cuda.synchronize()
+            b.run()
+            _check_batch(b, alone, "replay %d" % r)
+    finally:
+        b.close()
+
+
+def test_autotuned_batch(gc, dev, alone):
+    b = gc.Batch(GRIDS, use_graph=True)
+    try:
+        t = b.tuning()
+        print("tuned", t, flush=True)
+        assert t["brick_x"] in (64, 32, 16, 8)
+        assert 1 <= t["brick_lz"] <= 128
+        assert t["waves_per_cu"] in (8, 16, 32)
+        b.run()
+        _check_batch(b, alone, "tuned")
+    finally:
+        b.close()
+    # the second batch of the same signature takes the cached choice
+    b2 = gc.Batch(GRIDS, use_graph=False)
+    try:
+        assert b2.tuning()["brick_x"] == t["brick_x"]
+        assert b2.tuning()["brick_lz"] == t["brick_lz"]
+    finally:
+        b2.close()
+
+
+def test_batch_against_oracle(gc, orc, dev):
+    grids = [CF.plain_gyroid(40), CF.varying_schwarz_p(33)]
+    b = gc.Batch(grids, brick_x=8, brick_lz=4, waves_per_cu=4, use_graph=True)
+    try:
+        b.run()
+        res = b.results()
+        for i, p in enumerate(grids):
+            f = to_np(b.field(i, res))
+            ref = orc.lattice_field_f32(p)
+            assert np.array_equal(f.view(np.uint32), ref.view(np.uint32))
+            v, t, _ = orc.extract_indexed(f, p.nx, p.ny, p.nz)
+            m = b.mesh(i, res)
+            assert np.array_equal(to_np(m.vertices).view(np.uint32), v.view(np.uint32))
+            assert np.array_equal(to_np(m.triangles).astype(np.uint32), t)
+    finally:
+        b.close()
+
+
+def test_batch_overflow_reported(gc, dev, alone):
+    total_v = sum(len(a[1]) for a in alone)
+    total_t = sum(len(a[2]) for a in alone)
+    b = gc.Batch(GRIDS, brick_x=64, brick_lz=16, waves_per_cu=8, use_graph=False,
+                 vert_capacity=total_v // 2, tri_capacity=total_t)
+    try:
+        b.run()
+        with pytest.raises(gc.GpucadError) as e:
+            b.results()
+        assert e.value.status == gc.E_OVERFLOW
+        res = b.results(allow_overflow=True)
+        assert any(r["status"] == gc.E_OVERFLOW for r in res)
+        # the grids that fit are complete and exact
+        for i, r in enumerate(res):
+            if r["status"] == gc.OK:
+                m = b.mesh(i, res)
+                assert np.array_equal(to_np(m.triangles), alone[i][2])
+                assert np.array_equal(to_np(m.vertices).view(np.uint32),
+                                      alone[i][1].view(np.uint32))
+    finally:
+        b.close()
+
+
+def test_batch_bad_arguments(gc, dev):
+    with pytest.raises(gc.GpucadError):
+        gc.Batch([], use_graph=False)
+    with pytest.raises(gc.GpucadError):
+        gc.Batch([CF.plain_gyroid(32)], brick_x=24)
+    bad = CF.plain_gyroid(32)
+    bad.nx = 1
+    with pytest.raises(gc.GpucadError):
+        gc.Batch([CF.plain_gyroid(32), bad])
