@@ -31,6 +31,7 @@ OK, E_ARG, E_OVERFLOW, E_OOM, E_HIP, E_BUSY, E_NOTFOUND, E_IO = range(8)
 TPMS_GYROID, TPMS_SCHWARZ_P, TPMS_DIAMOND = 0, 1, 2
 PRIM_NONE, PRIM_SPHERE, PRIM_BOX = 0, 1, 2
 OP_NONE, OP_INTERSECT, OP_UNION, OP_SUBTRACT = 0, 1, 2, 3
+BOOL_UNION, BOOL_DIFF, BOOL_INTERSECT, BOOL_FIXED, BOOL_DYNAMIC, BOOL_MAKE_REGION = 1, 2, 4, 8, 16, 32
 MAX_MODES = 8
 
 
@@ -243,6 +244,14 @@ SIGNATURES = {
     "gpucad_compute_isosurface_standard": (
         [_vp, _vp, _vp, _f, _u32, _vp, _vp, _vp, _vp, Uint3, Uint3, Uint3, Float3, Float3,
          C.POINTER(_u32), C.POINTER(_u32), _vp, _u32], C.c_int),
+    "gpucad_classify_voxel_boolean": ([_vp, _vp, _vp, _vp, _vp, Uint3, Uint3, Uint3, _u32, _f, _f,
+                                       _f, _u32], C.c_int),
+    "gpucad_classify_copy_voxel": ([_vp, _vp, _vp, Uint3, _f, _f, _f, _u32], C.c_int),
+    "gpucad_generate_triangles_boolean": ([_vp, _vp, _vp, _vp, Uint3, Uint3, Uint3, Float3, Float3,
+                                           _f, _u32, _u32, _vp, _vp, _vp, _f, _f, _u32], C.c_int),
+    "gpucad_compute_isosurface": ([_vp, _vp, _f, _u32, _vp, _vp, _vp, _vp, Uint3, Uint3, Uint3,
+                                   Float3, Float3, C.POINTER(_u32), C.POINTER(_u32), _vp, _u32,
+                                   _vp, _vp, _vp, _f, _f, _u32, C.POINTER(_u64)], C.c_int),
     "gpucad_file_write_obj": ([_vp, _u32, C.c_char_p], C.c_int),
     "gpucad_write_obj_indexed": ([_vp, _u64, _vp, _u64, C.c_char_p, _i32], C.c_int),
 }

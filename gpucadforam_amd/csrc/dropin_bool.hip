@@ -1,0 +1,545 @@
+// Drop-in boolean meshing: the reference's computeIsosurface path for
+// primitives combined with each other or with a lattice band
+// (src/Isosurface.cu:44-134, src/MarchingCubes_kernel.cu:158-463 classify_copy_Voxel,
+// 869-1079 classifyVoxel, 1640-1784 vertexInterp_primitive/_primitive_one/_new,
+// 1865-2218 generateTriangles_lattice_kernel), writing the caller's uint arrays
+// and non-indexed float4 pos/norm exactly as the reference does.
+//
+// Fields: F = primitive_fixed (grid_points: val + per-point edge parameters
+// t_x, t_y, t_z of its +x/+y/+z edges), D = primitive_dynamic (float),
+// L = lattice field (float), band(L) = iso1 < L < iso2.  The corner
+// occupancy of each mode (flags, GC_BOOL_*):
+//   make_region            F < iso
+//   union     fixed        D < iso  |  band(L)
+//             dynamic      F < iso  |  band(L)
+//             (neither)    F < iso  |  D < iso
+//   diff      fixed        D >= iso &  band(L)
+//             dynamic      F < iso  &  !(iso1 <= L <= iso2)   (L < iso1 | L > iso2)
+//             (neither)    D >= iso &  F < iso
+//   intersect fixed        D < iso  &  band(L)
+//             dynamic      F < iso  &  band(L)
+//             (neither)    F < iso  &  D < iso
+// Vertices: make_region lerps with F's edge parameter; fixed uses
+// vertexInterp_new on (D, L); dynamic vertexInterp_primitive_one on L with F's
+// parameter; otherwise vertexInterp_primitive on D with F's parameter.  Face
+// normal cross(v2 - v0, v1 - v0), w = 0.5 (src/MarchingCubes_kernel.cu:2178-2191).
+//
+// Decided differences (the reference leaves these undefined): with none of
+// union/diff/intersect/make_region set the cube index is 0 (uninitialised in
+// the reference); every cell is classified (no 2-D grid fold); a volume the
+// mode does not read may be NULL.
+#include "gc_common.hpp"
+
+namespace gc {
+namespace {
+
+struct GridPoint {  // grid_points, src/MarchingCubes_kernel.h:12-18
+    int val;
+    float t_x, t_y, t_z;
+};
+static_assert(sizeof(GridPoint) == 16, "grid_points is 16 bytes");
+
+struct BGrid {
+    uint3 size, shift, mask;
+    float3 vs, center;
+};
+
+__device__ __forceinline__ uint3 calc_grid_pos(uint32_t i, uint3 shift, uint3 mask) {
+    // src/MarchingCubes_kernel.cu:120-136
+    uint32_t zq = i / shift.z;
+    uint32_t zr = i % shift.z;
+    return make_uint3(zr % mask.x, zr / mask.x, zq);
+}
+
+__device__ __forceinline__ uint32_t sample_index(uint3 p, uint3 gs) {
+    // src/MarchingCubes_kernel.cu:88-96, 99-118: clamped to gridSize
+    p.x = min(p.x, gs.x);
+    p.y = min(p.y, gs.y);
+    p.z = min(p.z, gs.z);
+    return p.z * gs.x * gs.y + p.y * gs.x + p.x;
+}
+
+__device__ __forceinline__ void corner_index(uint3 g, uint3 gs, uint32_t idx[8]) {
+    idx[0] = sample_index(g, gs);
+    idx[1] = sample_index(make_uint3(g.x + 1, g.y, g.z), gs);
+    idx[2] = sample_index(make_uint3(g.x + 1, g.y + 1, g.z), gs);
+    idx[3] = sample_index(make_uint3(g.x, g.y + 1, g.z), gs);
+    idx[4] = sample_index(make_uint3(g.x, g.y, g.z + 1), gs);
+    idx[5] = sample_index(make_uint3(g.x + 1, g.y, g.z + 1), gs);
+    idx[6] = sample_index(make_uint3(g.x + 1, g.y + 1, g.z + 1), gs);
+    idx[7] = sample_index(make_uint3(g.x, g.y + 1, g.z + 1), gs);
+}
+
+__device__ __forceinline__ bool in_band(float l, float lo, float hi) { return (l > lo) && (l < hi); }
+
+// occupancy of one corner (the table in the header)
+__device__ __forceinline__ uint32_t occupied(uint32_t flags, float fval, float d, float l,
+                                             float iso, float lo, float hi) {
+    const bool fixed = flags & GC_BOOL_FIXED, dynamic = flags & GC_BOOL_DYNAMIC;
+    if (flags & GC_BOOL_MAKE_REGION) return fval < iso;
+    if (flags & GC_BOOL_UNION) {
+        if (fixed) return (d < iso) | in_band(l, lo, hi);
+        if (dynamic) return (fval < iso) | in_band(l, lo, hi);
+        return (fval < iso) | (d < iso);
+    }
+    if (flags & GC_BOOL_DIFF) {
+        if (fixed) return (d >= iso) & in_band(l, lo, hi);
+        if (dynamic) return (fval < iso) & ((l < lo) | (l > hi));
+        return (d >= iso) & (fval < iso);
+    }
+    if (flags & GC_BOOL_INTERSECT) {
+        if (fixed) return (d < iso) & in_band(l, lo, hi);
+        if (dynamic) return (fval < iso) & in_band(l, lo, hi);
+        return (fval < iso) & (d < iso);
+    }
+    return 0;
+}
+
+struct Vols {
+    const GridPoint *fixed;
+    const float *dyn, *lat;
+};
+
+// which volumes a mode reads (fixed takes precedence over dynamic, as in the
+// reference's if / else-if chains)
+__host__ __device__ inline void needs(uint32_t flags, bool *f, bool *d, bool *l) {
+    *f = *d = *l = false;
+    if (flags & GC_BOOL_MAKE_REGION) {
+        *f = true;
+    } else if (flags & (GC_BOOL_UNION | GC_BOOL_DIFF | GC_BOOL_INTERSECT)) {
+        if (flags & GC_BOOL_FIXED) {
+            *d = *l = true;
+        } else if (flags & GC_BOOL_DYNAMIC) {
+            *f = *l = true;
+        } else {
+            *f = *d = true;
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t cube_index(uint32_t flags, const Vols &v, const uint32_t idx[8],
+                                               float iso, float lo, float hi, bool nf, bool nd,
+                                               bool nl) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float fv = nf ? (float)v.fixed[idx[k]].val : 0.f;
+        const float d = nd ? v.dyn[idx[k]] : 0.f;
+        const float l = nl ? v.lat[idx[k]] : 0.f;
+        c |= occupied(flags, fv, d, l, iso, lo, hi) << k;
+    }
+    return c;
+}
+
+__global__ void bool_classify_kernel(uint32_t *verts, uint32_t *occ, Vols v, BGrid g, uint32_t n,
+                                     float iso, float lo, float hi, uint32_t flags,
+                                     const uint8_t *nverts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bool nf, nd, nl;
+    needs(flags, &nf, &nd, &nl);
+    const uint3 p = calc_grid_pos(i, g.shift, g.mask);
+    uint32_t idx[8];
+    corner_index(p, g.size, idx);
+    const uint32_t nv = nverts[cube_index(flags, v, idx, iso, lo, hi, nf, nd, nl)];
+    verts[i] = nv;
+    occ[i] = nv > 0;
+}
+
+// ---- classify_copy_Voxel: occupancy as val = -1 / 1 and averaged edge parameters
+__device__ __forceinline__ bool crosses(float a, float b, float level) {
+    // (a < level && b >= level) || (a >= level && b < level), a = neighbour, b = own
+    return ((a < level) && (b >= level)) || ((a >= level) && (b < level));
+}
+
+__device__ __forceinline__ void blend(float &tp, float t) {
+    if (tp > 0) tp = (tp + t) * 0.5f;
+    else tp = t;
+}
+
+__global__ void bool_copy_kernel(GridPoint *vol_one, const float *vol_two, const float *vol_lat,
+                                 uint3 gs, float iso, float lo, float hi, uint32_t flags,
+                                 uint32_t n_points) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + 1 >= n_points) return;  // the reference stops before the last point
+    const uint32_t plane = gs.x * gs.y;
+    const uint3 p = make_uint3((i % plane) % gs.x, (i % plane) / gs.x, i / plane);
+    const bool dynamic = flags & GC_BOOL_DYNAMIC;
+    GridPoint g = vol_one[i];
+    const float v = dynamic ? 0.f : vol_two[sample_index(p, gs)];
+    const float vl = dynamic ? vol_lat[sample_index(p, gs)] : 0.f;
+    const float fv = (float)g.val;
+    bool inside;
+    bool set = true;
+    if (flags & GC_BOOL_UNION) {
+        inside = dynamic ? (in_band(vl, lo, hi) | (fv < iso)) : ((v < iso) | (fv < iso));
+    } else if (flags & GC_BOOL_DIFF) {
+        inside = dynamic ? (in_band(vl, lo, hi) & (fv >= iso)) : ((v >= iso) & (fv < iso));
+    } else if (flags & GC_BOOL_INTERSECT) {
+        inside = dynamic ? (in_band(vl, lo, hi) & (fv < iso)) : ((v < iso) & (fv < iso));
+    } else {
+        inside = false;
+        set = false;
+    }
+    if (set) g.val = inside ? -1 : 1;
+    const float own = dynamic ? vl : v;
+    const float *src = dynamic ? vol_lat : vol_two;
+    for (int ax = 0; ax < 3; ++ax) {
+        const uint32_t c = ax == 0 ? p.x : (ax == 1 ? p.y : p.z);
+        const uint32_t lim = ax == 0 ? gs.x : (ax == 1 ? gs.y : gs.z);
+        if (c >= lim - 1) continue;
+        const uint3 q = make_uint3(p.x + (ax == 0), p.y + (ax == 1), p.z + (ax == 2));
+        const float nb = src[sample_index(q, gs)];
+        float &tp = ax == 0 ? g.t_x : (ax == 1 ? g.t_y : g.t_z);
+        if (dynamic) {
+            if (crosses(nb, own, lo)) {
+                blend(tp, (lo - own) / (nb - own));
+            } else if (crosses(nb, own, hi)) {
+                blend(tp, (hi - own) / (nb - own));
+            }
+        } else if (crosses(nb, own, iso)) {
+            blend(tp, (iso - own) / (nb - own));
+        }
+    }
+    vol_one[i] = g;
+}
+
+// ---- generate
+__device__ __forceinline__ float3 lerp3(float3 a, float3 b, float t) {
+    return make_float3(__builtin_fmaf(t, b.x - a.x, a.x), __builtin_fmaf(t, b.y - a.y, a.y),
+                       __builtin_fmaf(t, b.z - a.z, a.z));
+}
+
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+    return make_float3(__builtin_fmaf(a.y, b.z, -(a.z * b.y)), __builtin_fmaf(a.z, b.x, -(a.x * b.z)),
+                       __builtin_fmaf(a.x, b.y, -(a.y * b.x)));
+}
+
+__device__ __forceinline__ bool straddles(float f0, float f1, float level) {
+    return ((f1 >= level) && (f0 <= level)) || ((f0 >= level) && (f1 <= level));
+}
+
+// merge of an edge parameter t1 with an interpolated t2 (vertexInterp_primitive)
+__device__ __forceinline__ float merge_t(float t1, float t2, float t) {
+    if ((t1 > 0) && (t2 > 0)) return (t1 + t2) * 0.5f;
+    if ((t1 > 0) && (t2 == 0)) return t1;
+    if ((t2 > 0) && (t1 == 0)) return t2;
+    return t;
+}
+
+// src/MarchingCubes_kernel.cu:1640-1671
+__device__ float3 interp_primitive(float iso, float3 p0, float3 p1, float f0, float f1, float te) {
+    float t2 = 0.0f;
+    if (straddles(f0, f1, iso)) t2 = (iso - f0) / (f1 - f0);
+    return lerp3(p0, p1, merge_t(te, t2, 0.0f));
+}
+
+// src/MarchingCubes_kernel.cu:1674-1726
+__device__ float3 interp_primitive_one(float lo, float hi, float3 p0, float3 p1, float f0,
+                                       float f1, float te) {
+    float t2 = 0.0f, t3 = 0.0f;
+    if (straddles(f0, f1, lo)) t2 = (lo - f0) / (f1 - f0);
+    float t = merge_t(te, t2, 0.0f);
+    if (straddles(f0, f1, hi)) t3 = (hi - f0) / (f1 - f0);
+    t = merge_t(te, t3, t);
+    return lerp3(p0, p1, t);
+}
+
+// src/MarchingCubes_kernel.cu:1729-1784
+__device__ float3 interp_new(float iso, float lo, float hi, float3 p0, float3 p1, float f0, float f1,
+                             float f2, float f3) {
+    float t1 = 0.0f, t2 = 0.0f, t3 = 0.0f, t = 0.0f;
+    if (((f0 < iso) && (f1 >= iso)) || ((f1 < iso) && (f0 >= iso))) t1 = (iso - f0) / (f1 - f0);
+    if (((f2 < lo) && (f3 >= lo)) || ((f3 < lo) && (f2 >= lo))) t2 = (lo - f2) / (f3 - f2);
+    if (((f2 < hi) && (f3 >= hi)) || ((f3 < hi) && (f2 >= hi))) t3 = (hi - f2) / (f3 - f2);
+    if ((t1 > 0.0f) && (t2 > 0.0f) && (t3 == 0.0f)) t = (t1 + t2) * 0.5f;
+    else if ((t1 > 0.0f) && (t3 > 0.0f) && (t2 == 0.0f)) t = (t1 + t3) * 0.5f;
+    else if ((t1 > 0.0f) && (t2 == 0.0f) && (t3 == 0.0f)) t = t1;
+    else if ((t2 > 0.0f) && (t1 == 0.0f) && (t3 == 0.0f)) t = t2;
+    else if ((t3 > 0.0f) && (t1 == 0.0f) && (t2 == 0.0f)) t = t3;
+    return lerp3(p0, p1, t);
+}
+
+// edges as the primitive modes read them: from corner a to corner b with the
+// edge parameter of corner c along axis ax (edge 2: corner 3 -> 2, edge 6: 7 -> 6)
+__constant__ int c_pa[12] = {0, 1, 3, 0, 4, 5, 7, 4, 0, 1, 2, 3};
+__constant__ int c_pb[12] = {1, 2, 2, 3, 5, 6, 6, 7, 4, 5, 6, 7};
+__constant__ int c_pax[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+// edges of vertexInterp_new (fixed mode): corner pairs in the cube's edge order
+__constant__ int c_na[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3};
+__constant__ int c_nb[12] = {1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7};
+
+struct BGenArgs {
+    float4 *pos, *norm;
+    const uint32_t *comp, *scan;
+    Vols v;
+    BGrid g;
+    float iso, lo, hi;
+    uint32_t active, max_verts, flags;
+    const uint8_t *tri;
+    const uint8_t *nverts;
+};
+
+__device__ __forceinline__ float tparam(const GridPoint &p, int ax) {
+    return ax == 0 ? p.t_x : (ax == 1 ? p.t_y : p.t_z);
+}
+
+__global__ void __launch_bounds__(64) bool_generate_kernel(BGenArgs a) {
+    __shared__ float3 vl[12][64];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.active) return;
+    const uint32_t voxel = a.comp[i];
+    const uint3 gp = calc_grid_pos(voxel, a.g.shift, a.g.mask);
+    float3 p;
+    p.x = ((float)gp.x - a.g.center.x) * a.g.vs.x;
+    p.y = ((float)gp.y - a.g.center.y) * a.g.vs.y;
+    p.z = ((float)gp.z - a.g.center.z) * a.g.vs.z;
+    const float3 vs = a.g.vs;
+    float3 v[8];
+    v[0] = p;
+    v[1] = make_float3(p.x + vs.x, p.y + 0.f, p.z + 0.f);
+    v[2] = make_float3(p.x + vs.x, p.y + vs.y, p.z + 0.f);
+    v[3] = make_float3(p.x + 0.f, p.y + vs.y, p.z + 0.f);
+    v[4] = make_float3(p.x + 0.f, p.y + 0.f, p.z + vs.z);
+    v[5] = make_float3(p.x + vs.x, p.y + 0.f, p.z + vs.z);
+    v[6] = make_float3(p.x + vs.x, p.y + vs.y, p.z + vs.z);
+    v[7] = make_float3(p.x + 0.f, p.y + vs.y, p.z + vs.z);
+    bool nf, nd, nl;
+    needs(a.flags, &nf, &nd, &nl);
+    uint32_t idx[8];
+    corner_index(gp, a.g.size, idx);
+    const uint32_t cube = cube_index(a.flags, a.v, idx, a.iso, a.lo, a.hi, nf, nd, nl);
+    const uint32_t nv = a.nverts[cube];
+    if (nv == 0) return;
+    const uint32_t f = a.flags;
+    if (f & GC_BOOL_MAKE_REGION) {
+        for (int e = 0; e < 12; ++e)
+            vl[e][threadIdx.x] =
+                lerp3(v[c_pa[e]], v[c_pb[e]], tparam(a.v.fixed[idx[c_pa[e]]], c_pax[e]));
+    } else if (f & GC_BOOL_FIXED) {
+        for (int e = 0; e < 12; ++e) {
+            const int ca = c_na[e], cb = c_nb[e];
+            vl[e][threadIdx.x] = interp_new(a.iso, a.lo, a.hi, v[ca], v[cb], a.v.dyn[idx[ca]],
+                                            a.v.dyn[idx[cb]], a.v.lat[idx[ca]], a.v.lat[idx[cb]]);
+        }
+    } else if (f & GC_BOOL_DYNAMIC) {
+        for (int e = 0; e < 12; ++e) {
+            const int ca = c_pa[e], cb = c_pb[e];
+            vl[e][threadIdx.x] =
+                interp_primitive_one(a.lo, a.hi, v[ca], v[cb], a.v.lat[idx[ca]], a.v.lat[idx[cb]],
+                                     tparam(a.v.fixed[idx[ca]], c_pax[e]));
+        }
+    } else {
+        for (int e = 0; e < 12; ++e) {
+            const int ca = c_pa[e], cb = c_pb[e];
+            vl[e][threadIdx.x] =
+                interp_primitive(a.iso, v[ca], v[cb], a.v.dyn[idx[ca]], a.v.dyn[idx[cb]],
+                                 tparam(a.v.fixed[idx[ca]], c_pax[e]));
+        }
+    }
+    const uint32_t base = a.scan[voxel];
+    for (uint32_t j = 0; j < nv; j += 3) {
+        const uint32_t index = base + j;
+        const float3 a0 = vl[a.tri[cube * 16 + j]][threadIdx.x];
+        const float3 a1 = vl[a.tri[cube * 16 + j + 1]][threadIdx.x];
+        const float3 a2 = vl[a.tri[cube * 16 + j + 2]][threadIdx.x];
+        // calcNormal(ver[0], ver[2], ver[1]) = cross(v2 - v0, v1 - v0)
+        const float3 e0 = make_float3(a2.x - a0.x, a2.y - a0.y, a2.z - a0.z);
+        const float3 e1 = make_float3(a1.x - a0.x, a1.y - a0.y, a1.z - a0.z);
+        const float3 n = cross3(e0, e1);
+        if (index < a.max_verts - 3u) {  // uint arithmetic as in the reference
+            a.pos[index] = make_float4(a0.x, a0.y, a0.z, 1.0f);
+            a.norm[index] = make_float4(n.x, n.y, n.z, 0.5f);
+            a.pos[index + 1] = make_float4(a1.x, a1.y, a1.z, 1.0f);
+            a.norm[index + 1] = make_float4(n.x, n.y, n.z, 0.5f);
+            a.pos[index + 2] = make_float4(a2.x, a2.y, a2.z, 1.0f);
+            a.norm[index + 2] = make_float4(n.x, n.y, n.z, 0.5f);
+        }
+    }
+}
+
+BGrid bgrid(gc_uint3 s, gc_uint3 sh, gc_uint3 m, gc_float3 vs, gc_float3 c) {
+    BGrid g;
+    g.size = make_uint3(s.x, s.y, s.z);
+    g.shift = make_uint3(sh.x, sh.y, sh.z);
+    g.mask = make_uint3(m.x, m.y, m.z);
+    g.vs = make_float3(vs.x, vs.y, vs.z);
+    g.center = make_float3(c.x, c.y, c.z);
+    return g;
+}
+
+int check_vols(uint32_t flags, const void *fixed, const float *dyn, const float *lat) {
+    bool nf, nd, nl;
+    needs(flags, &nf, &nd, &nl);
+    GC_ARG(!nf || fixed, "primitive_fixed is NULL but the mode reads it");
+    GC_ARG(!nd || dyn, "primitive_dynamic is NULL but the mode reads it");
+    GC_ARG(!nl || lat, "lattice_field is NULL but the mode reads it");
+    return GPUCAD_OK;
+}
+
+int bool_classify(uint32_t *verts, uint32_t *occ, const Vols &v, gc_uint3 s, gc_uint3 sh,
+                  gc_uint3 m, uint32_t n, float iso, float lo, float hi, uint32_t flags,
+                  hipStream_t st) {
+    GC_ARG(verts && occ, "NULL buffer");
+    int r = check_dropin_grid(s, sh, m, n);
+    if (r) return r;
+    if ((r = check_vols(flags, v.fixed, v.dyn, v.lat))) return r;
+    const DeviceTables *t = nullptr;
+    if ((r = device_tables(&t))) return r;
+    if (n == 0) return GPUCAD_OK;
+    const BGrid g = bgrid(s, sh, m, gc_float3{1, 1, 1}, gc_float3{0, 0, 0});
+    hipLaunchKernelGGL(bool_classify_kernel, dim3((n + 255) / 256), dim3(256), 0, st, verts, occ, v,
+                       g, n, iso, lo, hi, flags, t->num_verts);
+    GC_CHECK_LAUNCH();
+    return GPUCAD_OK;
+}
+
+int bool_generate(BGenArgs &a, hipStream_t st) {
+    GC_ARG(a.pos && a.norm && a.comp && a.scan, "NULL buffer");
+    int r = check_vols(a.flags, a.v.fixed, a.v.dyn, a.v.lat);
+    if (r) return r;
+    const DeviceTables *t = nullptr;
+    if ((r = device_tables(&t))) return r;
+    a.tri = t->tri_table;
+    a.nverts = t->num_verts;
+    if (a.active == 0) return GPUCAD_OK;
+    hipLaunchKernelGGL(bool_generate_kernel, dim3((a.active + 63) / 64), dim3(64), 0, st, a);
+    GC_CHECK_LAUNCH();
+    return GPUCAD_OK;
+}
+
+int read_u32(const uint32_t *d, uint32_t *h) {
+    GC_HIP(hipMemcpy(h, d, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return GPUCAD_OK;
+}
+
+}  // namespace
+}  // namespace gc
+
+using namespace gc;
+
+extern "C" {
+
+int gpucad_classify_voxel_boolean(uint32_t *voxel_verts, uint32_t *voxel_occupied,
+                                  const void *primitive_fixed, const float *primitive_dynamic,
+                                  const float *lattice_field, gc_uint3 grid_size,
+                                  gc_uint3 grid_size_shift, gc_uint3 grid_size_mask,
+                                  uint32_t num_voxels, float iso1, float iso2, float iso,
+                                  uint32_t flags) {
+    clear_error();
+    const Vols v = {(const GridPoint *)primitive_fixed, primitive_dynamic, lattice_field};
+    int r = bool_classify(voxel_verts, voxel_occupied, v, grid_size, grid_size_shift,
+                          grid_size_mask, num_voxels, iso, iso1, iso2, flags, 0);
+    if (r) return r;
+    GC_HIP(hipStreamSynchronize(0));
+    return GPUCAD_OK;
+}
+
+int gpucad_classify_copy_voxel(void *vol_one, const float *volume_two, const float *vol_lattice,
+                               gc_uint3 grid_size, float iso1, float iso2, float iso,
+                               uint32_t flags) {
+    clear_error();
+    GC_ARG(vol_one, "vol_one is NULL");
+    const bool dynamic = flags & GC_BOOL_DYNAMIC;
+    GC_ARG(dynamic ? vol_lattice != nullptr : volume_two != nullptr,
+           "the volume this mode reads is NULL");
+    const uint64_t n = (uint64_t)grid_size.x * grid_size.y * grid_size.z;
+    GC_ARG(grid_size.x >= 1 && grid_size.y >= 1 && grid_size.z >= 1 && n <= 0xffffffffull,
+           "bad grid size");
+    if (n < 2) return GPUCAD_OK;
+    hipLaunchKernelGGL(bool_copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0,
+                       (GridPoint *)vol_one, volume_two, vol_lattice,
+                       make_uint3(grid_size.x, grid_size.y, grid_size.z), iso, iso1, iso2, flags,
+                       (uint32_t)n);
+    GC_CHECK_LAUNCH();
+    GC_HIP(hipStreamSynchronize(0));
+    return GPUCAD_OK;
+}
+
+int gpucad_generate_triangles_boolean(gc_float4 *pos, gc_float4 *norm, const uint32_t *compacted,
+                                      const uint32_t *num_verts_scanned, gc_uint3 grid_size,
+                                      gc_uint3 grid_size_shift, gc_uint3 grid_size_mask,
+                                      gc_float3 voxel_size, gc_float3 grid_center, float iso,
+                                      uint32_t active_voxels, uint32_t max_verts,
+                                      const void *primitive_fixed,
+                                      const float *primitive_dynamic, const float *lattice_field,
+                                      float iso1, float iso2, uint32_t flags) {
+    clear_error();
+    int r = check_dropin_grid(grid_size, grid_size_shift, grid_size_mask, 0);
+    if (r) return r;
+    BGenArgs a;
+    a.pos = (float4 *)pos;
+    a.norm = (float4 *)norm;
+    a.comp = compacted;
+    a.scan = num_verts_scanned;
+    a.v = Vols{(const GridPoint *)primitive_fixed, primitive_dynamic, lattice_field};
+    a.g = bgrid(grid_size, grid_size_shift, grid_size_mask, voxel_size, grid_center);
+    a.iso = iso;
+    a.lo = iso1;
+    a.hi = iso2;
+    a.active = active_voxels;
+    a.max_verts = max_verts;
+    a.flags = flags;
+    if ((r = bool_generate(a, 0))) return r;
+    GC_HIP(hipStreamSynchronize(0));
+    return GPUCAD_OK;
+}
+
+int gpucad_compute_isosurface(gc_float4 *pos, gc_float4 *norm, float iso, uint32_t num_voxels,
+                              uint32_t *d_voxel_verts, uint32_t *d_voxel_verts_scan,
+                              uint32_t *d_voxel_occupied, uint32_t *d_voxel_occupied_scan,
+                              gc_uint3 grid_size, gc_uint3 grid_size_shift,
+                              gc_uint3 grid_size_mask, gc_float3 voxel_size,
+                              gc_float3 grid_center, uint32_t *active_voxels,
+                              uint32_t *total_verts, uint32_t *d_comp_voxel_array,
+                              uint32_t max_verts, const void *primitive_fixed,
+                              const float *primitive_dynamic, const float *lattice_field,
+                              float iso1, float iso2, uint32_t flags, uint64_t *nfacets) {
+    clear_error();
+    GC_ARG(active_voxels && total_verts && nfacets, "host outputs are NULL");
+    GC_ARG(num_voxels > 0, "numVoxels must be > 0");
+    GC_ARG(pos && norm && d_comp_voxel_array && d_voxel_verts_scan && d_voxel_occupied_scan,
+           "NULL buffer");
+    const Vols v = {(const GridPoint *)primitive_fixed, primitive_dynamic, lattice_field};
+    const uint32_t n = num_voxels;
+    int r = bool_classify(d_voxel_verts, d_voxel_occupied, v, grid_size, grid_size_shift,
+                          grid_size_mask, n, iso, iso1, iso2, flags, 0);
+    if (r) return r;
+    if ((r = dropin_scan_u32(d_voxel_occupied_scan, d_voxel_occupied, n, 0))) return r;
+    uint32_t last = 0, last_scan = 0;
+    if ((r = read_u32(d_voxel_occupied + (n - 1), &last))) return r;
+    if ((r = read_u32(d_voxel_occupied_scan + (n - 1), &last_scan))) return r;
+    *active_voxels = last + last_scan;
+    if (*active_voxels == 0) {  // src/Isosurface.cu:83-87 (nfacets left as it was)
+        *total_verts = 0;
+        return GPUCAD_OK;
+    }
+    if ((r = dropin_compact(d_comp_voxel_array, d_voxel_occupied, d_voxel_occupied_scan, n, 0)))
+        return r;
+    if ((r = dropin_scan_u32(d_voxel_verts_scan, d_voxel_verts, n, 0))) return r;
+    if ((r = read_u32(d_voxel_verts + (n - 1), &last))) return r;
+    if ((r = read_u32(d_voxel_verts_scan + (n - 1), &last_scan))) return r;
+    *total_verts = last + last_scan;
+    *nfacets = *total_verts / 3;
+    // src/Isosurface.cu:120-121 clears max_verts bytes (not elements) of each
+    GC_HIP(hipMemset(pos, 0, max_verts));
+    GC_HIP(hipMemset(norm, 0, max_verts));
+    BGenArgs a;
+    a.pos = (float4 *)pos;
+    a.norm = (float4 *)norm;
+    a.comp = d_comp_voxel_array;
+    a.scan = d_voxel_verts_scan;
+    a.v = v;
+    a.g = bgrid(grid_size, grid_size_shift, grid_size_mask, voxel_size, grid_center);
+    a.iso = iso;
+    a.lo = iso1;
+    a.hi = iso2;
+    a.active = *active_voxels;
+    a.max_verts = max_verts;
+    a.flags = flags;
+    if ((r = bool_generate(a, 0))) return r;
+    GC_HIP(hipStreamSynchronize(0));
+    return GPUCAD_OK;
+}
+
+}  // extern "C"

@@ -443,6 +443,18 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
 }
 
 }  // namespace
+
+int check_dropin_grid(gc_uint3 s, gc_uint3 sh, gc_uint3 m, uint32_t n) {
+    return check_grid(s, sh, m, n);
+}
+int dropin_scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st) {
+    return scan_u32(out, in, n, st);
+}
+int dropin_compact(uint32_t *comp, const uint32_t *occ, const uint32_t *occ_scan, uint32_t n,
+                   hipStream_t st) {
+    return compact(comp, occ, occ_scan, n, st);
+}
+
 }  // namespace gc
 
 using namespace gc;

@@ -556,6 +556,8 @@ __global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q
     __shared__ uint64_t s_tri[256];
     __shared__ uint16_t s_list[4][kListCap];
     __shared__ uint32_t s_vid[4][12][64];
+    __shared__ uint64_t s_pk[4][64];      // packed triangle list of each lane's cell
+    __shared__ uint16_t s_own[4][5 * 64]; // triangle -> lane | (triangle of the cell) << 8
     s_tri[threadIdx.x] = a.tri_packed[threadIdx.x];
     __syncthreads();
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -567,9 +569,14 @@ __global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q
         const uint64_t *signs = a.signs + s.sign_off;
         const uint64_t k = (uint64_t)(gb - s.block_off) * kBlock + tid;
         const uint32_t *base = a.base + (uint64_t)s.block_off * kBlock;  // by grid word
-        // ---- per word: sign words, crossing words, bases, active cells
-        uint64_t wa = 0, sa = 0, wb = 0, sb = 0, wc = 0, sc = 0, wd = 0, sd = 0, act = 0;
-        Cross c0 = {0, 0, 0, 0, 0}, c1 = c0, c2 = c0, c3 = c0;
+        // ---- per word (owner lane): the sign word of row (y, z) and the
+        // crossing words of the four rows (y..y+1, z..z+1) touching its cells;
+        // a cell's cube index and edge vertex numbers follow from these bits
+        // (sign of a neighbour corner = own sign ^ crossing of the edge between)
+        uint64_t wa = 0, act = 0;
+        uint64_t ex0 = 0, ey0 = 0, ez0 = 0, ex1 = 0, ey1 = 0, ez1 = 0;
+        uint64_t ex2 = 0, ey2 = 0, ez2 = 0, ex3 = 0, ey3 = 0, ez3 = 0;
+        uint32_t carry = 0;  // crossing bits of point 64 (next word): x0 y0 x1 y1 x2
         uint32_t vb0 = 0, vb1 = 0, vb2 = 0, vb3 = 0;
         if (k < g.words) {
             const WordPos p = word_pos(k, g);
@@ -578,18 +585,15 @@ __global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q
                 const int y = p.y, z = p.z, w = p.w;
                 wa = sword(signs, g, y, z, w);
                 const uint64_t na = sword(signs, g, y, z, w + 1);
-                wb = sword(signs, g, y + 1, z, w);
+                const uint64_t wb = sword(signs, g, y + 1, z, w);
                 const uint64_t nb = sword(signs, g, y + 1, z, w + 1);
-                wc = sword(signs, g, y, z + 1, w);
+                const uint64_t wc = sword(signs, g, y, z + 1, w);
                 const uint64_t nc = sword(signs, g, y, z + 1, w + 1);
-                wd = sword(signs, g, y + 1, z + 1, w);
+                const uint64_t wd = sword(signs, g, y + 1, z + 1, w);
                 const uint64_t nd = sword(signs, g, y + 1, z + 1, w + 1);
-                sa = shift1(wa, na);
-                sb = shift1(wb, nb);
-                sc = shift1(wc, nc);
-                sd = shift1(wd, nd);
                 const uint64_t ved = mask_below((int64_t)g.nx - 1 - 64 * w);
-                act = active_cells(wa, sa, wb, sb, wc, sc, wd, sd, ved);
+                act = active_cells(wa, shift1(wa, na), wb, shift1(wb, nb), wc, shift1(wc, nc), wd,
+                                   shift1(wd, nd), ved);
                 if (act) {
                     const uint64_t vpt = mask_below((int64_t)g.nx - 64 * w);
                     const bool vpt_n = 64 * (w + 1) < g.nx, ved_n = 64 * (w + 1) < g.nx - 1;
@@ -598,10 +602,17 @@ __global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q
                     const uint64_t wf = sword(signs, g, y + 2, z + 1, w);
                     const uint64_t wg = sword(signs, g, y, z + 2, w);
                     const uint64_t wh = sword(signs, g, y + 1, z + 2, w);
-                    c0 = row_cross(wa, na, wb, nb, wc, true, true, ved, vpt, ved_n, vpt_n);
-                    c1 = row_cross(wb, nb, we, ne, wd, has_y2, true, ved, vpt, ved_n, vpt_n);
-                    c2 = row_cross(wc, nc, wd, nd, wg, true, has_z2, ved, vpt, ved_n, vpt_n);
-                    c3 = row_cross(wd, nd, wf, 0ull, wh, has_y2, has_z2, ved, vpt, ved_n, vpt_n);
+                    const Cross c0 = row_cross(wa, na, wb, nb, wc, true, true, ved, vpt, ved_n, vpt_n);
+                    const Cross c1 = row_cross(wb, nb, we, ne, wd, has_y2, true, ved, vpt, ved_n, vpt_n);
+                    const Cross c2 = row_cross(wc, nc, wd, nd, wg, true, has_z2, ved, vpt, ved_n, vpt_n);
+                    const Cross c3 = row_cross(wd, nd, wf, 0ull, wh, has_y2, has_z2, ved, vpt, ved_n, vpt_n);
+                    ex0 = c0.ex; ey0 = c0.ey; ez0 = c0.ez;
+                    ex1 = c1.ex; ey1 = c1.ey; ez1 = c1.ez;
+                    ex2 = c2.ex; ey2 = c2.ey; ez2 = c2.ez;
+                    ex3 = c3.ex; ey3 = c3.ey; ez3 = c3.ez;
+                    carry = (uint32_t)(c0.sx >> 63) | ((uint32_t)(c0.sy >> 63) << 1) |
+                            ((uint32_t)(c1.sx >> 63) << 2) | ((uint32_t)(c1.sy >> 63) << 3) |
+                            ((uint32_t)(c2.sx >> 63) << 4);
                     const uint64_t rW = (uint64_t)g.W, rowW = (uint64_t)g.ny * g.W;
                     vb0 = base[k];
                     vb1 = base[k + rW];
@@ -618,6 +629,8 @@ __global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q
         uint64_t tnext = a.bsum[2 * (uint64_t)gb + 1];
         for (uint32_t i = 0; i < wave; ++i) tnext += a.twave[4 * (uint64_t)gb + i];
         uint32_t(*vid)[64] = s_vid[wave];
+        uint64_t *pk = s_pk[wave];
+        uint16_t *own = s_own[wave];
         for (uint32_t r0 = 0; r0 < atot; r0 += kListCap) {
             fill_list(list, act, nact, aex, r0, lane);
             const uint32_t n = min((uint32_t)kListCap, atot - r0);
@@ -626,39 +639,43 @@ __global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q
                 const bool on = e < n;
                 const uint32_t ent = on ? list[e] : 0u;
                 const uint32_t j = ent & 63, x = ent >> 6;
-                const uint32_t cube = cube_at(gather64(wa, j), gather64(sa, j), gather64(wb, j),
-                                              gather64(sb, j), gather64(wc, j), gather64(sc, j),
-                                              gather64(wd, j), gather64(sd, j), x);
+                const uint64_t gx0 = gather64(ex0, j), gy0 = gather64(ey0, j), gz0 = gather64(ez0, j);
+                const uint64_t gx1 = gather64(ex1, j), gy1 = gather64(ey1, j), gz1 = gather64(ez1, j);
+                const uint64_t gx2 = gather64(ex2, j), gy2 = gather64(ey2, j), gz2 = gather64(ez2, j);
+                const uint64_t gx3 = gather64(ex3, j), gy3 = gather64(ey3, j), gz3 = gather64(ez3, j);
+                const uint32_t a0 = bit_at(gather64(wa, j), x);
+                const uint32_t cy = gather32(carry, j);
+                // bits of point x+1 (the next word's point 0 when x = 63)
+                const bool last = x == 63;
+                const uint32_t xn = last ? 0u : x + 1;
+                const uint32_t sx0 = last ? (cy & 1u) : bit_at(gx0, xn);
+                const uint32_t sy0 = last ? ((cy >> 1) & 1u) : bit_at(gy0, xn);
+                const uint32_t sx1 = last ? ((cy >> 2) & 1u) : bit_at(gx1, xn);
+                const uint32_t sy1 = last ? ((cy >> 3) & 1u) : bit_at(gy1, xn);
+                const uint32_t sx2 = last ? ((cy >> 4) & 1u) : bit_at(gx2, xn);
+                const uint32_t x0 = bit_at(gx0, x), y0 = bit_at(gy0, x), z0 = bit_at(gz0, x);
+                const uint32_t x1 = bit_at(gx1, x), y1 = bit_at(gy1, x), z1 = bit_at(gz1, x);
+                const uint32_t x2 = bit_at(gx2, x), y2 = bit_at(gy2, x), z2 = bit_at(gz2, x);
+                const uint32_t x3 = bit_at(gx3, x);
+                // corners: 0 (x,y,z) 1 (x+1,y,z) 2 (x+1,y+1,z) 3 (x,y+1,z)
+                //          4 (x,y,z+1) 5 (x+1,y,z+1) 6 (x+1,y+1,z+1) 7 (x,y+1,z+1)
+                const uint32_t k0 = a0, k1 = a0 ^ x0, k3 = a0 ^ y0, k2 = k3 ^ x1;
+                const uint32_t k4 = a0 ^ z0, k5 = k4 ^ x2, k7 = k3 ^ z1, k6 = k7 ^ x3;
+                const uint32_t cube = k0 | (k1 << 1) | (k2 << 2) | (k3 << 3) | (k4 << 4) |
+                                      (k5 << 5) | (k6 << 6) | (k7 << 7);
                 const uint64_t packed = s_tri[cube];
                 const uint32_t nt = on ? (uint32_t)(packed >> 60) : 0u;
                 const uint32_t tincl = wave_scan_incl(nt);
-                const uint64_t t0 = tnext + (tincl - nt);
-                tnext += readlane32(tincl, 63);
+                const uint32_t tex = tincl - nt, ttot = readlane32(tincl, 63);
                 const uint64_t below = x ? (~0ull >> (64 - x)) : 0ull;
-                const uint64_t ex0 = gather64(c0.ex, j), ey0 = gather64(c0.ey, j),
-                               ez0 = gather64(c0.ez, j);
-                const uint64_t ex1 = gather64(c1.ex, j), ey1 = gather64(c1.ey, j),
-                               ez1 = gather64(c1.ez, j);
-                const uint64_t ex2 = gather64(c2.ex, j), ey2 = gather64(c2.ey, j),
-                               ez2 = gather64(c2.ez, j);
-                const uint64_t ex3 = gather64(c3.ex, j), ey3 = gather64(c3.ey, j),
-                               ez3 = gather64(c3.ez, j);
-                const uint32_t sx0 = bit_at(gather64(c0.sx, j), x),
-                               sy0 = bit_at(gather64(c0.sy, j), x);
-                const uint32_t sx1 = bit_at(gather64(c1.sx, j), x),
-                               sy1 = bit_at(gather64(c1.sy, j), x);
-                const uint32_t sx2 = bit_at(gather64(c2.sx, j), x);
-                const uint32_t p0 = gather32(vb0, j) + popc64(ex0 & below) +
-                                    popc64(ey0 & below) + popc64(ez0 & below);
-                const uint32_t p1 = gather32(vb1, j) + popc64(ex1 & below) +
-                                    popc64(ey1 & below) + popc64(ez1 & below);
-                const uint32_t p2 = gather32(vb2, j) + popc64(ex2 & below) +
-                                    popc64(ey2 & below) + popc64(ez2 & below);
-                const uint32_t p3 = gather32(vb3, j) + popc64(ex3 & below) +
-                                    popc64(ey3 & below) + popc64(ez3 & below);
-                const uint32_t x0 = bit_at(ex0, x), y0 = bit_at(ey0, x), z0 = bit_at(ez0, x);
-                const uint32_t x1 = bit_at(ex1, x), y1 = bit_at(ey1, x), z1 = bit_at(ez1, x);
-                const uint32_t x2 = bit_at(ex2, x), y2 = bit_at(ey2, x), z2 = bit_at(ez2, x);
+                const uint32_t p0 = gather32(vb0, j) + popc64(gx0 & below) + popc64(gy0 & below) +
+                                    popc64(gz0 & below);
+                const uint32_t p1 = gather32(vb1, j) + popc64(gx1 & below) + popc64(gy1 & below) +
+                                    popc64(gz1 & below);
+                const uint32_t p2 = gather32(vb2, j) + popc64(gx2 & below) + popc64(gy2 & below) +
+                                    popc64(gz2 & below);
+                const uint32_t p3 = gather32(vb3, j) + popc64(gx3 & below) + popc64(gy3 & below) +
+                                    popc64(gz3 & below);
                 const uint32_t q0 = p0 + x0 + y0 + z0;  // first edge of point x+1
                 const uint32_t q1 = p1 + x1 + y1 + z1;
                 const uint32_t q2 = p2 + x2 + y2 + z2;
@@ -675,18 +692,26 @@ __global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q
                     vid[9][lane] = q0 + sx0 + sy0;  // z-edge of (x+1, y,   z)
                     vid[10][lane] = q1 + sx1 + sy1; // z-edge of (x+1, y+1, z)
                     vid[11][lane] = p1 + x1 + y1;   // z-edge of (x,   y+1, z)
-                    for (uint32_t jt = 0; jt < nt; ++jt) {
-                        const uint64_t t = t0 + jt;
-                        if (t < a.tcap) {
-                            const uint32_t v0 = vid[(packed >> (12 * jt)) & 15][lane];
-                            const uint32_t v1 = vid[(packed >> (12 * jt + 4)) & 15][lane];
-                            const uint32_t v2 = vid[(packed >> (12 * jt + 8)) & 15][lane];
-                            *(T3 *)(a.triangles + 3 * t) = T3{v0, v1, v2};
-                        }
+                    pk[lane] = packed;
+                    for (uint32_t jt = 0; jt < nt; ++jt) own[tex + jt] = (uint16_t)(lane | (jt << 8));
+                }
+                __builtin_amdgcn_wave_barrier();
+                // the wave's triangles, one per lane: consecutive lanes store
+                // consecutive 12-byte records
+                for (uint32_t te = lane; te < ttot; te += 64) {
+                    const uint32_t o = own[te], l = o & 255, jt = o >> 8;
+                    const uint64_t pkl = pk[l];
+                    const uint64_t t = tnext + te;
+                    if (t < a.tcap) {
+                        const uint32_t v0 = vid[(pkl >> (12 * jt)) & 15][l];
+                        const uint32_t v1 = vid[(pkl >> (12 * jt + 4)) & 15][l];
+                        const uint32_t v2 = vid[(pkl >> (12 * jt + 8)) & 15][l];
+                        *(T3 *)(a.triangles + 3 * t) = T3{v0, v1, v2};
                     }
                 }
+                tnext += ttot;
+                __builtin_amdgcn_wave_barrier();
             }
-            __builtin_amdgcn_wave_barrier();
         }
     });
 }

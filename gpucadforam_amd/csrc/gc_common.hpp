@@ -53,6 +53,13 @@ const uint64_t *host_tri_packed();   // 256
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Shared pieces of the drop-in marching cubes (dropin_mc.hip): the grid
+// layout check, the exclusive uint32 scan and the compaction, enqueued on st.
+int check_dropin_grid(gc_uint3 s, gc_uint3 sh, gc_uint3 m, uint32_t n);
+int dropin_scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st);
+int dropin_compact(uint32_t *comp, const uint32_t *occ, const uint32_t *occ_scan, uint32_t n,
+                   hipStream_t st);
+
 // Scratch memory owned by the library for drop-in calls (grown on demand,
 // one arena per device, never shrunk).
 int scratch(size_t bytes, void **out);

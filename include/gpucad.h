@@ -450,6 +450,63 @@ int gpucad_compute_isosurface_standard(const float *vol, gc_float4 *pos, gc_floa
                                        uint32_t *d_comp_voxel_array, uint32_t max_verts);
 
 /* ------------------------------------------------------------------------ */
+/* Drop-in boolean meshing (primitives with each other or with a lattice band) */
+/* ------------------------------------------------------------------------ */
+/* The reference's boolean switches as one bit set (bools of
+ * Isosurface::computeIsosurface, src/Isosurface.h:28-33).  primitive_fixed
+ * points to grid_points records {int val; float t_x, t_y, t_z;} (16 bytes). */
+enum {
+    GC_BOOL_UNION = 1,
+    GC_BOOL_DIFF = 2,
+    GC_BOOL_INTERSECT = 4,
+    GC_BOOL_FIXED = 8,
+    GC_BOOL_DYNAMIC = 16,
+    GC_BOOL_MAKE_REGION = 32
+};
+/* MarchingCubeCuda::classifyVoxel_lattice (src/MarchingCubes_kernel.h,
+ * kernel classifyVoxel src/MarchingCubes_kernel.cu:869-1047): cube index from
+ * the boolean occupancy of each corner. */
+int gpucad_classify_voxel_boolean(uint32_t *voxel_verts, uint32_t *voxel_occupied,
+                                  const void *primitive_fixed, const float *primitive_dynamic,
+                                  const float *lattice_field, gc_uint3 grid_size,
+                                  gc_uint3 grid_size_shift, gc_uint3 grid_size_mask,
+                                  uint32_t num_voxels, float iso1, float iso2, float iso,
+                                  uint32_t flags);
+/* MarchingCubeCuda::classify_copy_Voxel_lattice (kernel classify_copy_Voxel,
+ * src/MarchingCubes_kernel.cu:158-446): rewrites vol_one's val as -1 (inside
+ * the boolean) / 1 and blends the crossing parameters of each point's +x/+y/+z
+ * edges into t_x/t_y/t_z; grid_size = points per axis; every point but the
+ * last. */
+int gpucad_classify_copy_voxel(void *vol_one, const float *volume_two, const float *vol_lattice,
+                               gc_uint3 grid_size, float iso1, float iso2, float iso,
+                               uint32_t flags);
+/* MarchingCubeCuda::generateTriangles_lattice (kernel
+ * generateTriangles_lattice_kernel, src/MarchingCubes_kernel.cu:1865-2199):
+ * non-indexed pos/norm, normal cross(v2 - v0, v1 - v0) with w = 0.5. */
+int gpucad_generate_triangles_boolean(gc_float4 *pos, gc_float4 *norm, const uint32_t *compacted,
+                                      const uint32_t *num_verts_scanned, gc_uint3 grid_size,
+                                      gc_uint3 grid_size_shift, gc_uint3 grid_size_mask,
+                                      gc_float3 voxel_size, gc_float3 grid_center, float iso,
+                                      uint32_t active_voxels, uint32_t max_verts,
+                                      const void *primitive_fixed,
+                                      const float *primitive_dynamic, const float *lattice_field,
+                                      float iso1, float iso2, uint32_t flags);
+/* Isosurface::computeIsosurface (src/Isosurface.h:28-33, src/Isosurface.cu:44-134):
+ * classify, scans, compaction and generate; fills *active_voxels,
+ * *total_verts and *nfacets = total/3 (left unchanged when nothing is
+ * active, as the reference does). */
+int gpucad_compute_isosurface(gc_float4 *pos, gc_float4 *norm, float iso, uint32_t num_voxels,
+                              uint32_t *d_voxel_verts, uint32_t *d_voxel_verts_scan,
+                              uint32_t *d_voxel_occupied, uint32_t *d_voxel_occupied_scan,
+                              gc_uint3 grid_size, gc_uint3 grid_size_shift,
+                              gc_uint3 grid_size_mask, gc_float3 voxel_size,
+                              gc_float3 grid_center, uint32_t *active_voxels,
+                              uint32_t *total_verts, uint32_t *d_comp_voxel_array,
+                              uint32_t max_verts, const void *primitive_fixed,
+                              const float *primitive_dynamic, const float *lattice_field,
+                              float iso1, float iso2, uint32_t flags, uint64_t *nfacets);
+
+/* ------------------------------------------------------------------------ */
 /* .obj output                                                               */
 /* ------------------------------------------------------------------------ */
 /* File_output::file_write_obj (src/File_output.h:189, src/File_output.cu:5-81):

@@ -5,7 +5,7 @@
 // src/main.cu) compiles against them unchanged and runs the MI355X kernels:
 //
 //   MarchingCubeCuda   src/MarchingCubes_kernel.h:34-124 (path methods)
-//   Isosurface         src/Isosurface.h:10-80 (lattice extraction methods)
+//   Isosurface         src/Isosurface.h:10-80 (lattice and boolean extraction)
 //   Modelling          src/Modelling.h:10-48 (primitive fields)
 //   Fft_lattice        src/lattice_files/Fft_lattice.h:10-24 (create_lattice)
 //   Gratings           src/lattice_files/Gratings.h:10-80 (normalise, masks)
@@ -49,6 +49,13 @@ inline void check(int status, const char *what) {
 
 inline gc_uint3 u3(uint3 v) { return gc_uint3{v.x, v.y, v.z}; }
 inline gc_float3 f3(float3 v) { return gc_float3{v.x, v.y, v.z}; }
+
+inline uint32_t bool_flags(bool obj_union, bool obj_diff, bool obj_intersect, bool fixed,
+                           bool dynamic, bool make_region) {
+    return (obj_union ? GC_BOOL_UNION : 0u) | (obj_diff ? GC_BOOL_DIFF : 0u) |
+           (obj_intersect ? GC_BOOL_INTERSECT : 0u) | (fixed ? GC_BOOL_FIXED : 0u) |
+           (dynamic ? GC_BOOL_DYNAMIC : 0u) | (make_region ? GC_BOOL_MAKE_REGION : 0u);
+}
 
 inline void ensure_init() {
     static bool done = false;
@@ -142,10 +149,97 @@ public:
                   isovalue1, isovalue2),
               "generateTriangles_lattice_newone");
     }
+
+    // src/MarchingCubes_kernel.h:39-45 (boolean meshing stages); primitive,
+    // topo and compute_lattice select nothing in these kernels
+    void classify_copy_Voxel_lattice(dim3, dim3, uint *, grid_points *vol_one, float *volume_two,
+                                     float *vol_lattice, bool fixed, bool dynamic, float iso1,
+                                     float iso2, uint3 gridSize, uint3, uint3, uint, float3,
+                                     float isoValue, bool obj_union, bool obj_diff,
+                                     bool obj_intersect) {
+        using namespace gpucad_compat;
+        ensure_init();
+        check(gpucad_classify_copy_voxel(
+                  vol_one, volume_two, vol_lattice, u3(gridSize), iso1, iso2, isoValue,
+                  bool_flags(obj_union, obj_diff, obj_intersect, fixed, dynamic, false)),
+              "classify_copy_Voxel_lattice");
+    }
+
+    void classifyVoxel_lattice(dim3, dim3, float *, uint3, uint *voxelVerts, uint *voxelOccupied,
+                               grid_points *primitive_fixed, float *primitive_dynamic, float *,
+                               float *lattice_field, uint3 gridSize, uint3 gridSizeShift,
+                               uint3 gridSizeMask, uint numVoxels, float iso1, float iso2, float3,
+                               float isoValue, bool obj_union, bool obj_diff, bool obj_intersect,
+                               bool, bool, bool, bool fixed, bool dynamic, bool make_region) {
+        using namespace gpucad_compat;
+        ensure_init();
+        check(gpucad_classify_voxel_boolean(
+                  voxelVerts, voxelOccupied, primitive_fixed, primitive_dynamic, lattice_field,
+                  u3(gridSize), u3(gridSizeShift), u3(gridSizeMask), numVoxels, iso1, iso2,
+                  isoValue,
+                  bool_flags(obj_union, obj_diff, obj_intersect, fixed, dynamic, make_region)),
+              "classifyVoxel_lattice");
+    }
+
+    void generateTriangles_lattice(dim3, dim3, float4 *pos, float4 *norm, uint *compactVoxelArray,
+                                   uint *numVertsScanned, uint3 gridSize, uint3 gridSizeShift,
+                                   uint3 gridSizeMask, float3 voxelSize, float3 gridcenter,
+                                   float isoValue, uint activeVoxels, uint maxVerts, uint,
+                                   grid_points *primitive_fixed, float *primitive_dynamic,
+                                   float *, float *lattice_field, float iso1, float iso2, uint *,
+                                   bool obj_union, bool obj_diff, bool obj_intersect, bool, bool,
+                                   bool, bool fixed, bool dynamic, bool make_region) {
+        using namespace gpucad_compat;
+        ensure_init();
+        check(gpucad_generate_triangles_boolean(
+                  (gc_float4 *)pos, (gc_float4 *)norm, compactVoxelArray, numVertsScanned,
+                  u3(gridSize), u3(gridSizeShift), u3(gridSizeMask), f3(voxelSize),
+                  f3(gridcenter), isoValue, activeVoxels, maxVerts, primitive_fixed,
+                  primitive_dynamic, lattice_field, iso1, iso2,
+                  bool_flags(obj_union, obj_diff, obj_intersect, fixed, dynamic, make_region)),
+              "generateTriangles_lattice");
+    }
 };
 
 class Isosurface : public MarchingCubeCuda {
 public:
+    // src/Isosurface.h:14-16, src/Isosurface.cu:15-27
+    void copy_parameter(uint *voxel_verts, float isoValue, uint3 gridSize, uint3 gridSizeShift,
+                        uint3 gridSizeMask, float3 voxelSize, uint numVoxels,
+                        grid_points *vol_one, float *vol_two, float *vol_lattice, bool fixed,
+                        bool dynamic, float iso1, float iso2, bool obj_union, bool obj_diff,
+                        bool obj_intersect) {
+        classify_copy_Voxel_lattice(dim3(), dim3(), voxel_verts, vol_one, vol_two, vol_lattice,
+                                    fixed, dynamic, iso1, iso2, gridSize, gridSizeShift,
+                                    gridSizeMask, numVoxels, voxelSize, isoValue, obj_union,
+                                    obj_diff, obj_intersect);
+    }
+
+    // src/Isosurface.h:28-33, src/Isosurface.cu:44-134
+    void computeIsosurface(float *, uint3, float4 *pos, float4 *norm, float isoValue,
+                           uint numVoxels, uint *d_voxelVerts, uint *d_voxelVertsScan,
+                           uint *d_voxelOccupied, uint *d_voxelOccupiedScan, uint3 gridSize,
+                           uint3 gridSizeShift, uint3 gridSizeMask, float3 voxelSize,
+                           float3 gridcenter, uint *activeVoxels, uint *totalVerts,
+                           uint *d_compVoxelArray, uint maxVerts, grid_points *primitive_fixed,
+                           float *primitive_dynamic, float *, float *lattice_field, float iso1,
+                           float iso2, bool obj_union, bool obj_diff, bool obj_intersect, bool,
+                           bool, bool, bool fixed, bool dynamic, bool make_region,
+                           size_t *nfacets) {
+        using namespace gpucad_compat;
+        ensure_init();
+        uint64_t nf = nfacets ? (uint64_t)*nfacets : 0;
+        check(gpucad_compute_isosurface(
+                  (gc_float4 *)pos, (gc_float4 *)norm, isoValue, numVoxels, d_voxelVerts,
+                  d_voxelVertsScan, d_voxelOccupied, d_voxelOccupiedScan, u3(gridSize),
+                  u3(gridSizeShift), u3(gridSizeMask), f3(voxelSize), f3(gridcenter),
+                  activeVoxels, totalVerts, d_compVoxelArray, maxVerts, primitive_fixed,
+                  primitive_dynamic, lattice_field, iso1, iso2,
+                  bool_flags(obj_union, obj_diff, obj_intersect, fixed, dynamic, make_region),
+                  &nf),
+              "computeIsosurface");
+        if (nfacets) *nfacets = (size_t)nf;
+    }
     void computeIsosurface_lattice(float *vol, float4 *pos, float4 *norm, float &isoValue,
                                    uint numVoxels, uint *d_voxelVerts, uint *d_voxelVertsScan,
                                    uint *d_voxelOccupied, uint *d_voxelOccupiedScan,

@@ -605,6 +605,148 @@ void generate(int mode, const float *vol, const float *vol1, const float *vol2, 
     }
 }
 
+// --- boolean meshing (Isosurface::computeIsosurface) ------------------------
+// grid_points (src/MarchingCubes_kernel.h:12-18)
+struct GPt {
+    int val;
+    float t_x, t_y, t_z;
+};
+enum { B_UNION = 1, B_DIFF = 2, B_INTERSECT = 4, B_FIXED = 8, B_DYNAMIC = 16, B_REGION = 32 };
+
+// corner occupancy of classifyVoxel (src/MarchingCubes_kernel.cu:869-1047):
+// F = fixed grid_points .val (as float), D = dynamic float, L = lattice float
+int bool_corner(unsigned flags, float F, float D, float L, float iso, float iso1, float iso2) {
+    const int band = (int)(L > iso1) & (int)(L < iso2);
+    if (flags & B_REGION) return F < iso;
+    if (flags & B_UNION) {
+        if (flags & B_FIXED) return (int)(D < iso) | band;
+        if (flags & B_DYNAMIC) return (int)(F < iso) | band;
+        return (int)(F < iso) | (int)(D < iso);
+    }
+    if (flags & B_DIFF) {
+        if (flags & B_FIXED) return (int)(D >= iso) & band;
+        if (flags & B_DYNAMIC) return (int)(F < iso) & ((int)(L < iso1) | (int)(L > iso2));
+        return (int)(D >= iso) & (int)(F < iso);
+    }
+    if (flags & B_INTERSECT) {
+        if (flags & B_FIXED) return (int)(D < iso) & band;
+        if (flags & B_DYNAMIC) return (int)(F < iso) & band;
+        return (int)(F < iso) & (int)(D < iso);
+    }
+    return 0;  // the reference leaves the cube index uninitialised here
+}
+
+int bool_cube(unsigned flags, const GPt *fx, const float *dy, const float *lt, uint32_t x,
+              uint32_t y, uint32_t z, const Grid &g, float iso, float iso1, float iso2) {
+    int cube = 0;
+    for (int k = 0; k < 8; ++k) {
+        uint32_t px = std::min(x + kCorner[k][0], g.sx), py = std::min(y + kCorner[k][1], g.sy),
+                 pz = std::min(z + kCorner[k][2], g.sz);
+        uint32_t i = pz * g.sx * g.sy + py * g.sx + px;
+        float F = fx ? (float)fx[i].val : 0.f, D = dy ? dy[i] : 0.f, L = lt ? lt[i] : 0.f;
+        cube |= bool_corner(flags, F, D, L, iso, iso1, iso2) << k;
+    }
+    return cube;
+}
+
+// vertexInterp_primitive (src/MarchingCubes_kernel.cu:1640-1671)
+F3 vi_primitive(float iso, F3 p0, F3 p1, float f0, float f1, float edge_id) {
+    float t1 = edge_id, t2 = 0.0f, t = 0.0f;
+    if (((f1 >= iso) && (f0 <= iso)) || ((f0 >= iso) && (f1 <= iso))) t2 = (iso - f0) / (f1 - f0);
+    if ((t1 > 0) && (t2 > 0)) t = (t1 + t2) * 0.5f;
+    else if ((t1 > 0) && (t2 == 0)) t = t1;
+    else if ((t2 > 0) && (t1 == 0)) t = t2;
+    return lerp3(p0, p1, t);
+}
+
+// vertexInterp_primitive_one (src/MarchingCubes_kernel.cu:1674-1726)
+F3 vi_primitive_one(float lo, float hi, F3 p0, F3 p1, float f0, float f1, float edge_id) {
+    float t1 = edge_id, t2 = 0.0f, t3 = 0.0f, t = 0.0f;
+    if (((f1 >= lo) && (f0 <= lo)) || ((f0 >= lo) && (f1 <= lo))) t2 = (lo - f0) / (f1 - f0);
+    if ((t1 > 0) && (t2 > 0)) t = (t1 + t2) * 0.5f;
+    else if ((t1 > 0) && (t2 == 0)) t = t1;
+    else if ((t2 > 0) && (t1 == 0)) t = t2;
+    if (((f1 >= hi) && (f0 <= hi)) || ((f0 >= hi) && (f1 <= hi))) t3 = (hi - f0) / (f1 - f0);
+    if ((t1 > 0) && (t3 > 0)) t = (t1 + t3) * 0.5f;
+    else if ((t1 > 0) && (t3 == 0)) t = t1;
+    else if ((t3 > 0) && (t1 == 0)) t = t3;
+    return lerp3(p0, p1, t);
+}
+
+// vertexInterp_new (src/MarchingCubes_kernel.cu:1729-1784)
+F3 vi_new(float iso, float lo, float hi, F3 p0, F3 p1, float f0, float f1, float f2, float f3) {
+    float t1 = 0.0f, t2 = 0.0f, t3 = 0.0f, t = 0.0f;
+    if (((f0 < iso) && (f1 >= iso)) || ((f1 < iso) && (f0 >= iso))) t1 = (iso - f0) / (f1 - f0);
+    if (((f2 < lo) && (f3 >= lo)) || ((f3 < lo) && (f2 >= lo))) t2 = (lo - f2) / (f3 - f2);
+    if (((f2 < hi) && (f3 >= hi)) || ((f3 < hi) && (f2 >= hi))) t3 = (hi - f2) / (f3 - f2);
+    if ((t1 > 0.0f) && (t2 > 0.0f) && (t3 == 0.0f)) t = (t1 + t2) * 0.5f;
+    else if ((t1 > 0.0f) && (t3 > 0.0f) && (t2 == 0.0f)) t = (t1 + t3) * 0.5f;
+    else if ((t1 > 0.0f) && (t2 == 0.0f) && (t3 == 0.0f)) t = t1;
+    else if ((t2 > 0.0f) && (t1 == 0.0f) && (t3 == 0.0f)) t = t2;
+    else if ((t3 > 0.0f) && (t1 == 0.0f) && (t2 == 0.0f)) t = t3;
+    return lerp3(p0, p1, t);
+}
+
+// generateTriangles_lattice_kernel (src/MarchingCubes_kernel.cu:1865-2199)
+void bool_generate(unsigned flags, const GPt *fx, const float *dy, const float *lt, const Grid &g,
+                   float iso, float iso1, float iso2, const uint32_t *comp, const uint32_t *scan,
+                   uint32_t active, uint32_t max_verts, float *pos, float *norm) {
+    // the primitive modes' edges: (start corner, end corner, axis of the start
+    // corner's edge parameter); vertexInterp_new walks the plain edge list
+    static const int pe[12][3] = {{0, 1, 0}, {1, 2, 1}, {3, 2, 0}, {0, 3, 1}, {4, 5, 0}, {5, 6, 1},
+                                  {7, 6, 0}, {4, 7, 1}, {0, 4, 2}, {1, 5, 2}, {2, 6, 2}, {3, 7, 2}};
+    for (uint32_t i = 0; i < active; ++i) {
+        const uint32_t voxel = comp[i];
+        uint32_t gx, gy, gz;
+        grid_pos(voxel, g, gx, gy, gz);
+        F3 p{((float)gx - g.cx) * g.vsx, ((float)gy - g.cy) * g.vsy, ((float)gz - g.cz) * g.vsz};
+        F3 v[8];
+        uint32_t ci[8];
+        for (int k = 0; k < 8; ++k) {
+            v[k] = F3{p.x + (kCorner[k][0] ? g.vsx : 0.f), p.y + (kCorner[k][1] ? g.vsy : 0.f),
+                      p.z + (kCorner[k][2] ? g.vsz : 0.f)};
+            uint32_t px = std::min(gx + kCorner[k][0], g.sx), py = std::min(gy + kCorner[k][1], g.sy),
+                     pz = std::min(gz + kCorner[k][2], g.sz);
+            ci[k] = pz * g.sx * g.sy + py * g.sx + px;
+        }
+        const int cube = bool_cube(flags, fx, dy, lt, gx, gy, gz, g, iso, iso1, iso2);
+        const int nv = T.nverts[cube];
+        if (nv == 0) continue;
+        F3 vl[12];
+        for (int e = 0; e < 12; ++e) {
+            const int a = pe[e][0], b = pe[e][1], ax = pe[e][2];
+            float te = 0.f;
+            if (fx) te = ax == 0 ? fx[ci[a]].t_x : (ax == 1 ? fx[ci[a]].t_y : fx[ci[a]].t_z);
+            if (flags & B_REGION) {
+                vl[e] = lerp3(v[a], v[b], te);
+            } else if (flags & B_FIXED) {
+                const int na = kEdge[e][0], nb = kEdge[e][1];
+                vl[e] = vi_new(iso, iso1, iso2, v[na], v[nb], dy[ci[na]], dy[ci[nb]], lt[ci[na]],
+                               lt[ci[nb]]);
+            } else if (flags & B_DYNAMIC) {
+                vl[e] = vi_primitive_one(iso1, iso2, v[a], v[b], lt[ci[a]], lt[ci[b]], te);
+            } else {
+                vl[e] = vi_primitive(iso, v[a], v[b], dy[ci[a]], dy[ci[b]], te);
+            }
+        }
+        for (int j = 0; j < nv; j += 3) {
+            const uint32_t index = scan[voxel] + j;
+            F3 a0 = vl[T.tri[cube][j]], a1 = vl[T.tri[cube][j + 1]], a2 = vl[T.tri[cube][j + 2]];
+            // calcNormal(ver[0], ver[2], ver[1])
+            F3 n = cross3(F3{a2.x - a0.x, a2.y - a0.y, a2.z - a0.z},
+                          F3{a1.x - a0.x, a1.y - a0.y, a1.z - a0.z});
+            if (index < max_verts - 3u) {
+                const F3 vv[3] = {a0, a1, a2};
+                for (int k = 0; k < 3; ++k) {
+                    float *P = pos + 4 * (uint64_t)(index + k), *N = norm + 4 * (uint64_t)(index + k);
+                    P[0] = vv[k].x; P[1] = vv[k].y; P[2] = vv[k].z; P[3] = 1.0f;
+                    N[0] = n.x; N[1] = n.y; N[2] = n.z; N[3] = 0.5f;
+                }
+            }
+        }
+    }
+}
+
 // --- reference primitives (src/Modelling.cu:244-761), FP32 as restated -----
 struct G3 {
     int nx, ny, nz;
@@ -768,6 +910,85 @@ int orc_compute_isosurface(int mode, const float *vol, const float *vol1, const 
     orc_generate(mode, vol, vol1, vol2, nx, ny, nz, voxel_size, center, iso, lo1, hi1, lo2, hi2,
                  comp, verts_scan, *active, max_verts, pos, norm);
     return 0;
+}
+
+// ---- boolean meshing entry points (grid layout as orc_generate: gridSize =
+// points, cells = points - 1 per axis)
+void orc_bool_classify(unsigned flags, const void *fixed, const float *dyn, const float *lat,
+                       int nx, int ny, int nz, float iso, float iso1, float iso2, uint32_t *verts,
+                       uint32_t *occ) {
+    Grid g{(uint32_t)nx, (uint32_t)ny, (uint32_t)nz, (uint32_t)((nx - 1) * (ny - 1)),
+           (uint32_t)(nx - 1), 1, 1, 1, 0, 0, 0};
+    const uint64_t n = (uint64_t)(nx - 1) * (ny - 1) * (nz - 1);
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t x, y, z;
+        grid_pos((uint32_t)i, g, x, y, z);
+        const int c = bool_cube(flags, (const GPt *)fixed, dyn, lat, x, y, z, g, iso, iso1, iso2);
+        verts[i] = T.nverts[c];
+        occ[i] = T.nverts[c] > 0;
+    }
+}
+
+// classify_copy_Voxel (src/MarchingCubes_kernel.cu:158-446)
+void orc_bool_copy(unsigned flags, void *vol_one, const float *vol_two, const float *vol_lat,
+                   int nx, int ny, int nz, float iso, float iso1, float iso2) {
+    GPt *vo = (GPt *)vol_one;
+    const uint64_t n = (uint64_t)nx * ny * nz;
+    const bool dynamic = flags & B_DYNAMIC;
+    for (uint64_t i = 0; i + 1 < n; ++i) {
+        const uint32_t plane = (uint32_t)(nx * ny);
+        const uint32_t z = (uint32_t)(i / plane), rem = (uint32_t)(i % plane);
+        const uint32_t y = rem / nx, x = rem % nx;
+        GPt p = vo[i];
+        const float v = dynamic ? 0.f : vol_two[i], vl = dynamic ? vol_lat[i] : 0.f;
+        const int band = (int)(vl > iso1) & (int)(vl < iso2);
+        if (flags & B_UNION) {
+            p.val = (dynamic ? (band | (int)(p.val < iso)) : ((int)(v < iso) | (int)(p.val < iso)))
+                        ? -1 : 1;
+        } else if (flags & B_DIFF) {
+            p.val = (dynamic ? (band & (int)(p.val >= iso)) : ((int)(v >= iso) & (int)(p.val < iso)))
+                        ? -1 : 1;
+        } else if (flags & B_INTERSECT) {
+            p.val = (dynamic ? (band & (int)(p.val < iso)) : ((int)(v < iso) & (int)(p.val < iso)))
+                        ? -1 : 1;
+        }
+        const uint32_t coord[3] = {x, y, z};
+        const uint32_t lim[3] = {(uint32_t)nx, (uint32_t)ny, (uint32_t)nz};
+        const uint64_t step[3] = {1, (uint64_t)nx, (uint64_t)nx * ny};
+        float *tp[3] = {&p.t_x, &p.t_y, &p.t_z};
+        for (int ax = 0; ax < 3; ++ax) {
+            if (coord[ax] >= lim[ax] - 1) continue;
+            if (dynamic) {
+                const float q = vol_lat[i + step[ax]];
+                if ((q < iso1 && vl >= iso1) || (q >= iso1 && vl < iso1)) {
+                    const float t = (iso1 - vl) / (q - vl);
+                    *tp[ax] = *tp[ax] > 0 ? (*tp[ax] + t) * 0.5f : t;
+                } else if ((q < iso2 && vl >= iso2) || (q >= iso2 && vl < iso2)) {
+                    const float t = (iso2 - vl) / (q - vl);
+                    *tp[ax] = *tp[ax] > 0 ? (*tp[ax] + t) * 0.5f : t;
+                }
+            } else {
+                const float q = vol_two[i + step[ax]];
+                if ((q < iso && v >= iso) || (q >= iso && v < iso)) {
+                    const float t = (iso - v) / (q - v);
+                    *tp[ax] = *tp[ax] > 0 ? (*tp[ax] + t) * 0.5f : t;
+                }
+            }
+        }
+        vo[i] = p;
+    }
+}
+
+void orc_bool_generate(unsigned flags, const void *fixed, const float *dyn, const float *lat,
+                       int nx, int ny, int nz, const float *voxel_size, const float *center,
+                       float iso, float iso1, float iso2, const uint32_t *comp,
+                       const uint32_t *scan, uint32_t active, uint32_t max_verts, float *pos,
+                       float *norm) {
+    Grid g{(uint32_t)nx, (uint32_t)ny, (uint32_t)nz, (uint32_t)((nx - 1) * (ny - 1)),
+           (uint32_t)(nx - 1), voxel_size[0], voxel_size[1], voxel_size[2], center[0], center[1],
+           center[2]};
+    bool_generate(flags, (const GPt *)fixed, dyn, lat, g, iso, iso1, iso2, comp, scan, active,
+                  max_verts, pos, norm);
 }
 
 // ---- Gratings::GPU_buffer_normalise_four (src/lattice_files/Gratings.cu:1089-1134,

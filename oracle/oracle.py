@@ -57,6 +57,13 @@ def _load():
                            C.c_float, f32p, f32p, f32p, C.c_int, C.c_int], None),
         "orc_file_write_obj": ([f32p, C.c_uint32, C.c_char_p], C.c_int),
         "orc_max_threads": ([], C.c_int),
+        "orc_bool_classify": ([C.c_uint, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_float,
+                               C.c_float, C.c_float, u32p, u32p], None),
+        "orc_bool_copy": ([C.c_uint, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                           C.c_float], None),
+        "orc_bool_generate": ([C.c_uint, vp, vp, vp, C.c_int, C.c_int, C.c_int, f32p, f32p,
+                               C.c_float, C.c_float, C.c_float, u32p, u32p, C.c_uint32,
+                               C.c_uint32, f32p, f32p], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -214,3 +221,68 @@ def primitive(kind, nx, ny, nz, d, center, angles, prm, flag=0, nthreads=None):
 def file_write_obj(pos4, total, filename):
     pos4 = np.ascontiguousarray(pos4, np.float32).reshape(-1)
     return lib.orc_file_write_obj(pos4, total, filename.encode())
+
+
+# ---------------------------------------------------------------------------
+# boolean meshing (Isosurface::computeIsosurface and its stages)
+# ---------------------------------------------------------------------------
+B_UNION, B_DIFF, B_INTERSECT, B_FIXED, B_DYNAMIC, B_REGION = 1, 2, 4, 8, 16, 32
+GRID_POINT = np.dtype([("val", np.int32), ("t_x", np.float32), ("t_y", np.float32),
+                       ("t_z", np.float32)])
+
+
+def _ptr(a, dtype):
+    if a is None:
+        return None, None
+    a = np.ascontiguousarray(a, dtype)
+    return a, a.ctypes.data
+
+
+def bool_classify(flags, fixed, dyn, lat, nx, ny, nz, iso, iso1, iso2):
+    n = (nx - 1) * (ny - 1) * (nz - 1)
+    verts = np.empty(n, np.uint32)
+    occ = np.empty(n, np.uint32)
+    kf, pf = _ptr(fixed, GRID_POINT)
+    kd, pd = _ptr(dyn, np.float32)
+    kl, pl = _ptr(lat, np.float32)
+    lib.orc_bool_classify(flags, pf, pd, pl, nx, ny, nz, iso, iso1, iso2, verts, occ)
+    return verts, occ
+
+
+def bool_copy(flags, vol_one, vol_two, vol_lat, nx, ny, nz, iso, iso1, iso2):
+    """classify_copy_Voxel on a copy of vol_one (grid_points); returns the copy."""
+    out = np.array(vol_one, dtype=GRID_POINT, copy=True)
+    k2, p2 = _ptr(vol_two, np.float32)
+    kl, pl = _ptr(vol_lat, np.float32)
+    lib.orc_bool_copy(flags, out.ctypes.data, p2, pl, nx, ny, nz, iso, iso1, iso2)
+    return out
+
+
+def bool_compute_isosurface(flags, fixed, dyn, lat, nx, ny, nz, iso, iso1, iso2,
+                            voxel_size=(1, 1, 1), center=(0, 0, 0), max_verts=None):
+    """The whole computeIsosurface sequence (classify, scans, compact, generate)."""
+    n = (nx - 1) * (ny - 1) * (nz - 1)
+    if max_verts is None:
+        max_verts = max(nx * ny * nz * 4, 300000)
+    verts, occ = bool_classify(flags, fixed, dyn, lat, nx, ny, nz, iso, iso1, iso2)
+    occ_scan = exclusive_scan_u32(occ)
+    active = int(occ[-1]) + int(occ_scan[-1])
+    out = {"verts": verts, "occ": occ, "occ_scan": occ_scan, "active": active}
+    pos = np.zeros(4 * max_verts, np.float32)
+    norm = np.zeros(4 * max_verts, np.float32)
+    if active == 0:
+        out.update(total=0, pos=pos.reshape(-1, 4), norm=norm.reshape(-1, 4))
+        return out
+    comp = np.zeros(n, np.uint32)
+    lib.orc_compact(occ, occ_scan, n, comp)
+    verts_scan = exclusive_scan_u32(verts)
+    total = int(verts[-1]) + int(verts_scan[-1])
+    kf, pf = _ptr(fixed, GRID_POINT)
+    kd, pd = _ptr(dyn, np.float32)
+    kl, pl = _ptr(lat, np.float32)
+    lib.orc_bool_generate(flags, pf, pd, pl, nx, ny, nz, np.asarray(voxel_size, np.float32),
+                          np.asarray(center, np.float32), iso, iso1, iso2, comp, verts_scan,
+                          active, max_verts, pos, norm)
+    out.update(comp=comp, verts_scan=verts_scan, total=total, nfacets=total // 3,
+               pos=pos.reshape(-1, 4), norm=norm.reshape(-1, 4))
+    return out
