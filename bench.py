@@ -39,6 +39,19 @@ PEAK_HBM_GBS = 8000.0
 FLOP_PER_POINT = {"varying_schwarz_p": 297, "plain_gyroid": 82, "bent_diamond": 144}
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per dispatch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary (profiles/hbm_traffic_per_dispatch.txt), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_per_dispatch.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 6 and not line.startswith("#") and f[0] == kernel and f[1] == workload:
+            return int(round((float(f[2]) * float(f[4]) + float(f[3]) * float(f[5])) * 1024))
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +142,37 @@ def cpu_baseline(p, slab, threads, budget_s=12.0):
                   "%.1f s total): oracle field %.3f s + indexed extraction %.3f s per slab"
                   % (len(runs), slab + 1, p.nx, p.ny, p.nz, pts, spent, tf, te),
     }
+
+
+def fp64_errors(p, field_gpu, z0, planes):
+    """Max |error| of the GPU's FP32 field and of its interpolated vertices
+    against FP64 evaluation (the oracle's double-precision twin of the field,
+    and the crossing parameter recomputed in double from it) on the z-slab
+    [z0, z0 + planes) of the workload.  Vertex error in units of the point
+    spacing; only edges whose sign pattern agrees in FP32 and FP64 compare."""
+    import numpy as np
+    from oracle import oracle as orc
+    nx, ny = p.nx, p.ny
+    plane = nx * ny
+    f32 = field_gpu[z0 * plane:(z0 + planes) * plane].cpu().numpy()
+    f64 = orc.lattice_field_f64(p, z0, z0 + planes)
+    field_err = float(np.max(np.abs(f32.astype(np.float64) - f64)))
+    verts, _, eids = orc.extract_indexed(f32, nx, ny, planes, spacing=(1.0, 1.0, 1.0),
+                                         edge_ids=True)
+    eids = eids.astype(np.int64)
+    pt, ax = eids // 3, eids % 3
+    step = np.array([1, nx, plane], np.int64)[ax]
+    a, b = f64[pt], f64[pt + step]
+    same = (a < 0) != (b < 0)
+    t = (0.0 - a[same]) / (b[same] - a[same])
+    x, y, z = pt % nx, (pt // nx) % ny, pt // plane
+    ref = np.stack([x, y, z], 1).astype(np.float64)[same]
+    ref[np.arange(len(t)), ax[same]] += t
+    dv = np.max(np.abs(verts[same].astype(np.float64) - ref), axis=1) if len(t) else np.zeros(1)
+    return {"slab": "z %d..%d of %dx%dx%d" % (z0, z0 + planes - 1, nx, ny, p.nz),
+            "field_max_abs": field_err, "vertex_max_abs_voxels": float(dv.max()),
+            "vertex_p99_abs_voxels": float(np.percentile(dv, 99)),
+            "vertices_compared": int(len(t)), "vertices_sign_differs": int((~same).sum())}
 
 
 def percentile(xs, q):
@@ -295,9 +339,14 @@ def main():
                                "sheet, intersect sphere r=0.45; indexed extraction"
                                % (p.nx, p.ny, p.nz),
                    "grid": [p.nx, p.ny, p.nz]},
-        "roofline": {"bound": "mfma" if False else "valu", "achieved": round(field_tflops, 2),
+        # the field kernel is the longest kernel of the step and is bound by
+        # the FP32 vector ALUs (no matrix work on this path): its roofline is the
+        # FP32 vector peak, flops counted by the per-point contract above
+        "roofline": {"bound": "valu", "achieved": round(field_tflops, 2),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(field_tflops / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "frac": round(field_tflops / PEAK_FP32_TFLOPS, 4),
+                     "traffic": measured_traffic("lattice_field_kernel",
+                                                 "varying_schwarz_p_%d" % p.nx),
                      "kernel": "lattice_field_kernel",
                      "flop_per_point": FLOP_PER_POINT["varying_schwarz_p"],
                      "launch_ms": round(field_ms, 4)},
@@ -309,6 +358,8 @@ def main():
         "n_triangles": counts[1],
         "wall_s": round(wall, 3),
     }
+    if not args.no_cpu:
+        rec["fp64_error"] = fp64_errors(p, pipe.field, p.nz // 2 - 16, 33)
     del pipe
     torch.cuda.empty_cache()
     if not args.no_batch:
