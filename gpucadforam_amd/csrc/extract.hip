@@ -130,8 +130,10 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) {
     return q;
 }
 
-__device__ __forceinline__ uint64_t sword(const uint64_t *s, const Grid &g, int y, int z, int w) {
-    return (y < g.ny && z < g.nz && w < g.W) ? s[((uint64_t)z * g.ny + y) * g.W + w] : 0ull;
+// sign word `idx` of a grid (32-bit word index: a grid has fewer than 2^32
+// words) when `ok`, else 0
+__device__ __forceinline__ uint64_t ldw(const uint64_t *s, uint32_t idx, bool ok) {
+    return ok ? s[idx] : 0ull;
 }
 
 struct WordPos {
@@ -293,12 +295,14 @@ __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a, Que
         if (k < g.words) {
             const WordPos p = word_pos(k, g);
             const bool has_y = p.y + 1 < g.ny, has_z = p.z + 1 < g.nz;
-            wa = sword(signs, g, p.y, p.z, p.w);
-            const uint64_t an = sword(signs, g, p.y, p.z, p.w + 1);
-            wb = sword(signs, g, p.y + 1, p.z, p.w);
-            const uint64_t bn = sword(signs, g, p.y + 1, p.z, p.w + 1);
-            wc = sword(signs, g, p.y, p.z + 1, p.w);
-            const uint64_t cn = sword(signs, g, p.y, p.z + 1, p.w + 1);
+            const uint32_t kk = (uint32_t)k, W = (uint32_t)g.W, RW = (uint32_t)g.ny * W;
+            const bool hw = p.w + 1 < g.W;
+            wa = signs[kk];
+            const uint64_t an = ldw(signs, kk + 1, hw);
+            wb = ldw(signs, kk + W, has_y);
+            const uint64_t bn = ldw(signs, kk + W + 1, has_y && hw);
+            wc = ldw(signs, kk + RW, has_z);
+            const uint64_t cn = ldw(signs, kk + RW + 1, has_z && hw);
             sa = shift1(wa, an);
             sb = shift1(wb, bn);
             sc = shift1(wc, cn);
@@ -309,8 +313,8 @@ __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a, Que
             const uint64_t ez = has_z ? (wa ^ wc) & vpt : 0ull;
             vc = popc64(ex) + popc64(ey) + popc64(ez);
             if (has_y && has_z) {
-                wd = sword(signs, g, p.y + 1, p.z + 1, p.w);
-                sd = shift1(wd, sword(signs, g, p.y + 1, p.z + 1, p.w + 1));
+                wd = signs[kk + RW + W];
+                sd = shift1(wd, ldw(signs, kk + RW + W + 1, hw));
                 act = active_cells(wa, sa, wb, sb, wc, sc, wd, sd, ved);
             }
         }
@@ -358,44 +362,50 @@ __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a, Que
 __global__ void __launch_bounds__(1024)
 scan_blocks_kernel(uint64_t *bsum, uint64_t nb, uint64_t *ctl, uint64_t *d_counts,
                    const Seg *segs, uint32_t nseg, SegResult *res) {
-    // thread t owns the strided entries t, t + 1024, ... so that every pass of
-    // the loop is one coalesced load; the scan then runs over (pass, thread)
+    // thread t owns the contiguous entries [t*per, (t+1)*per): it sums them,
+    // the workgroup scans the 1024 sums, then the thread writes its entries'
+    // exclusive prefixes (one pass over the block sums, one barrier round)
     __shared__ uint64_t sw_v[16], sw_t[16];
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const ulonglong2 *in = (const ulonglong2 *)bsum;
-    ulonglong2 *out = (ulonglong2 *)bsum;
-    uint64_t carry_v = 0, carry_t = 0;
-    for (uint64_t b0 = 0; b0 < nb; b0 += 1024) {
-        const uint64_t i = b0 + t;
-        const ulonglong2 q = i < nb ? in[i] : make_ulonglong2(0, 0);
-        uint64_t iv = q.x, it = q.y;  // inclusive scan within the wave
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint64_t pv = __shfl_up(iv, off, 64), pt = __shfl_up(it, off, 64);
-            if (lane >= (uint32_t)off) {
-                iv += pv;
-                it += pt;
-            }
-        }
-        if (lane == 63) {
-            sw_v[wave] = iv;
-            sw_t[wave] = it;
-        }
-        __syncthreads();
-        uint64_t wv = carry_v, wt = carry_t, tv = carry_v, tt = carry_t;
-        for (uint32_t w = 0; w < 16; ++w) {
-            if (w < wave) {
-                wv += sw_v[w];
-                wt += sw_t[w];
-            }
-            tv += sw_v[w];
-            tt += sw_t[w];
-        }
-        if (i < nb) out[i] = make_ulonglong2(wv + iv - q.x, wt + it - q.y);
-        carry_v = tv;
-        carry_t = tt;
-        __syncthreads();
+    ulonglong2 *io = (ulonglong2 *)bsum;
+    const uint64_t per = (nb + 1023) / 1024;
+    const uint64_t b0 = min(nb, (uint64_t)t * per), b1 = min(nb, b0 + per);
+    uint64_t sv = 0, st = 0;
+    for (uint64_t i = b0; i < b1; ++i) {
+        const ulonglong2 q = io[i];
+        sv += q.x;
+        st += q.y;
     }
+    uint64_t iv = sv, it = st;  // inclusive scan within the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t pv = __shfl_up(iv, off, 64), pt = __shfl_up(it, off, 64);
+        if (lane >= (uint32_t)off) {
+            iv += pv;
+            it += pt;
+        }
+    }
+    if (lane == 63) {
+        sw_v[wave] = iv;
+        sw_t[wave] = it;
+    }
+    __syncthreads();
+    uint64_t rv = iv - sv, rt = it - st, carry_v = 0, carry_t = 0;
+    for (uint32_t w = 0; w < 16; ++w) {
+        if (w < wave) {
+            rv += sw_v[w];
+            rt += sw_t[w];
+        }
+        carry_v += sw_v[w];
+        carry_t += sw_t[w];
+    }
+    for (uint64_t i = b0; i < b1; ++i) {
+        const ulonglong2 q = io[i];
+        io[i] = make_ulonglong2(rv, rt);
+        rv += q.x;
+        rt += q.y;
+    }
+    __syncthreads();
     // every thread holds the totals; the scanned sums are visible to the workgroup
     int any_ovf = 0;
     for (uint32_t i = t; i < nseg; i += 1024) {
@@ -457,13 +467,13 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a, Queue
         if (k < g.words) {
             const WordPos p = word_pos(k, g);
             const bool has_y = p.y + 1 < g.ny, has_z = p.z + 1 < g.nz;
-            const uint64_t w0 = sword(signs, g, p.y, p.z, p.w),
-                           w0n = sword(signs, g, p.y, p.z, p.w + 1);
+            const uint32_t kk = (uint32_t)k, W = (uint32_t)g.W, RW = (uint32_t)g.ny * W;
+            const uint64_t w0 = signs[kk], w0n = ldw(signs, kk + 1, p.w + 1 < g.W);
             const uint64_t vpt = mask_below((int64_t)g.nx - 64 * p.w);
             const uint64_t ved = mask_below((int64_t)g.nx - 1 - 64 * p.w);
             ex = (w0 ^ shift1(w0, w0n)) & ved;
-            ey = has_y ? (w0 ^ sword(signs, g, p.y + 1, p.z, p.w)) & vpt : 0ull;
-            ez = has_z ? (w0 ^ sword(signs, g, p.y, p.z + 1, p.w)) & vpt : 0ull;
+            ey = has_y ? (w0 ^ signs[kk + W]) & vpt : 0ull;
+            ez = has_z ? (w0 ^ signs[kk + RW]) & vpt : 0ull;
             s_p0[tid] = ((uint64_t)p.z * g.ny + p.y) * g.nx + 64 * (uint64_t)p.w;
             s_xyz[tid] = (uint32_t)(64 * p.w) | ((uint32_t)p.y << 16);
             s_z[tid] = (uint32_t)p.z;
@@ -582,26 +592,28 @@ __global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q
             const WordPos p = word_pos(k, g);
             if (p.y + 1 < g.ny && p.z + 1 < g.nz) {
                 const bool has_y2 = p.y + 2 < g.ny, has_z2 = p.z + 2 < g.nz;
-                const int y = p.y, z = p.z, w = p.w;
-                wa = sword(signs, g, y, z, w);
-                const uint64_t na = sword(signs, g, y, z, w + 1);
-                const uint64_t wb = sword(signs, g, y + 1, z, w);
-                const uint64_t nb = sword(signs, g, y + 1, z, w + 1);
-                const uint64_t wc = sword(signs, g, y, z + 1, w);
-                const uint64_t nc = sword(signs, g, y, z + 1, w + 1);
-                const uint64_t wd = sword(signs, g, y + 1, z + 1, w);
-                const uint64_t nd = sword(signs, g, y + 1, z + 1, w + 1);
+                const int w = p.w;
+                const uint32_t kk = (uint32_t)k, W = (uint32_t)g.W, RW = (uint32_t)g.ny * W;
+                const bool hw = w + 1 < g.W;
+                wa = signs[kk];
+                const uint64_t na = ldw(signs, kk + 1, hw);
+                const uint64_t wb = signs[kk + W];
+                const uint64_t nb = ldw(signs, kk + W + 1, hw);
+                const uint64_t wc = signs[kk + RW];
+                const uint64_t nc = ldw(signs, kk + RW + 1, hw);
+                const uint64_t wd = signs[kk + RW + W];
+                const uint64_t nd = ldw(signs, kk + RW + W + 1, hw);
                 const uint64_t ved = mask_below((int64_t)g.nx - 1 - 64 * w);
                 act = active_cells(wa, shift1(wa, na), wb, shift1(wb, nb), wc, shift1(wc, nc), wd,
                                    shift1(wd, nd), ved);
                 if (act) {
                     const uint64_t vpt = mask_below((int64_t)g.nx - 64 * w);
                     const bool vpt_n = 64 * (w + 1) < g.nx, ved_n = 64 * (w + 1) < g.nx - 1;
-                    const uint64_t we = sword(signs, g, y + 2, z, w),
-                                   ne = sword(signs, g, y + 2, z, w + 1);
-                    const uint64_t wf = sword(signs, g, y + 2, z + 1, w);
-                    const uint64_t wg = sword(signs, g, y, z + 2, w);
-                    const uint64_t wh = sword(signs, g, y + 1, z + 2, w);
+                    const uint64_t we = ldw(signs, kk + 2 * W, has_y2),
+                                   ne = ldw(signs, kk + 2 * W + 1, has_y2 && hw);
+                    const uint64_t wf = ldw(signs, kk + 2 * W + RW, has_y2);
+                    const uint64_t wg = ldw(signs, kk + 2 * RW, has_z2);
+                    const uint64_t wh = ldw(signs, kk + 2 * RW + W, has_z2);
                     const Cross c0 = row_cross(wa, na, wb, nb, wc, true, true, ved, vpt, ved_n, vpt_n);
                     const Cross c1 = row_cross(wb, nb, we, ne, wd, has_y2, true, ved, vpt, ved_n, vpt_n);
                     const Cross c2 = row_cross(wc, nc, wd, nd, wg, true, has_z2, ved, vpt, ved_n, vpt_n);
@@ -613,11 +625,10 @@ __global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q
                     carry = (uint32_t)(c0.sx >> 63) | ((uint32_t)(c0.sy >> 63) << 1) |
                             ((uint32_t)(c1.sx >> 63) << 2) | ((uint32_t)(c1.sy >> 63) << 3) |
                             ((uint32_t)(c2.sx >> 63) << 4);
-                    const uint64_t rW = (uint64_t)g.W, rowW = (uint64_t)g.ny * g.W;
-                    vb0 = base[k];
-                    vb1 = base[k + rW];
-                    vb2 = base[k + rowW];
-                    vb3 = base[k + rowW + rW];
+                    vb0 = base[kk];
+                    vb1 = base[kk + W];
+                    vb2 = base[kk + RW];
+                    vb3 = base[kk + RW + W];
                 }
             }
         }
