@@ -103,15 +103,19 @@ __device__ __forceinline__ uint32_t readlane32(uint32_t v, uint32_t l) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
 }
 
-// inclusive scan over the 64 lanes of a wave
+// inclusive scan over the 64 lanes of a wave with DPP moves (no LDS): shifts
+// by 1, 2, 4, 8 inside each row of 16 lanes, then the last lane of row 0 / 2
+// is broadcast into rows 1 / 3 (row_bcast:15) and the last lane of row 1 into
+// rows 2 and 3 (row_bcast:31).  Lanes without a source add the `old` operand, 0.
 __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t t = (uint32_t)__shfl_up((int)v, off, 64);
-        if (lane >= (uint32_t)off) v += t;
-    }
-    return v;
+    int x = (int)v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return (uint32_t)x;
 }
 
 // 12-byte records with 4-byte alignment (stored as one dwordx3 where possible)
@@ -439,6 +443,27 @@ scan_blocks_kernel(uint64_t *bsum, uint64_t nb, uint64_t *ctl, uint64_t *d_count
 // ---------------------------------------------------------------------------
 // vertices (+ per-word first vertex numbers)
 // ---------------------------------------------------------------------------
+// vertex on the edge from point (xf, yf, zf) along axis ax with end values f0, f1:
+// t = (iso - f0)/(f1 - f0), p = fma(t, p1 - p0, p0) on that axis
+__device__ __forceinline__ V3 edge_vertex(const Seg &s, float xf, float yf, float zf, uint32_t ax,
+                                          float f0, float f1) {
+    const float t = (s.iso - f0) / (f1 - f0);
+    float px = __builtin_fmaf(xf, s.sx, s.ox);
+    float py = __builtin_fmaf(yf, s.sy, s.oy);
+    float pz = __builtin_fmaf(zf, s.sz, s.oz);
+    if (ax == 0) {
+        const float p1 = __builtin_fmaf(xf + 1.0f, s.sx, s.ox);
+        px = __builtin_fmaf(t, p1 - px, px);
+    } else if (ax == 1) {
+        const float p1 = __builtin_fmaf(yf + 1.0f, s.sy, s.oy);
+        py = __builtin_fmaf(t, p1 - py, py);
+    } else {
+        const float p1 = __builtin_fmaf(zf + 1.0f, s.sz, s.oz);
+        pz = __builtin_fmaf(t, p1 - pz, pz);
+    }
+    return V3{px, py, pz};
+}
+
 struct VertArgs {
     const float *field;     // field buffer base
     const uint64_t *signs;  // sign buffer base
@@ -515,32 +540,31 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a, Queue
             }
             __syncthreads();
             const uint32_t n = min((uint32_t)kVertCap, vtot - r0);
-            for (uint32_t e = tid; e < n; e += kBlock) {
-                const uint32_t ent = s_list[e];
-                const uint32_t j = ent >> 8, b = (ent >> 2) & 63, ax = ent & 3;
-                const uint64_t pidx = s_p0[j] + b;
-                const uint32_t xy = s_xyz[j];
-                const float xf = (float)((xy & 0xffff) + b);
-                const float yf = (float)(xy >> 16), zf = (float)s_z[j];
-                const uint64_t step = ax == 0 ? 1ull : (ax == 1 ? (uint64_t)g.nx : plane);
-                const float f0 = field[pidx];
-                const float f1 = field[pidx + step];
-                const float t = (s.iso - f0) / (f1 - f0);
-                float px = __builtin_fmaf(xf, s.sx, s.ox);
-                float py = __builtin_fmaf(yf, s.sy, s.oy);
-                float pz = __builtin_fmaf(zf, s.sz, s.oz);
-                if (ax == 0) {
-                    const float p1 = __builtin_fmaf(xf + 1.0f, s.sx, s.ox);
-                    px = __builtin_fmaf(t, p1 - px, px);
-                } else if (ax == 1) {
-                    const float p1 = __builtin_fmaf(yf + 1.0f, s.sy, s.oy);
-                    py = __builtin_fmaf(t, p1 - py, py);
-                } else {
-                    const float p1 = __builtin_fmaf(zf + 1.0f, s.sz, s.oz);
-                    pz = __builtin_fmaf(t, p1 - pz, pz);
+            // two edges per thread and step, so that four field loads are in
+            // flight before the first interpolation
+            for (uint32_t e = tid; e < n; e += 2 * kBlock) {
+                const uint32_t e2 = e + kBlock;
+                const bool two = e2 < n;
+                const uint32_t ea = s_list[e], eb = s_list[two ? e2 : e];
+                const uint32_t ja = ea >> 8, ba = (ea >> 2) & 63, axa = ea & 3;
+                const uint32_t jb = eb >> 8, bb = (eb >> 2) & 63, axb = eb & 3;
+                const uint64_t pa = s_p0[ja] + ba, pb = s_p0[jb] + bb;
+                const uint64_t sa = axa == 0 ? 1ull : (axa == 1 ? (uint64_t)g.nx : plane);
+                const uint64_t sb = axb == 0 ? 1ull : (axb == 1 ? (uint64_t)g.nx : plane);
+                const float f0a = field[pa], f1a = field[pa + sa];
+                const float f0b = field[pb], f1b = field[pb + sb];
+                const uint32_t xya = s_xyz[ja], xyb = s_xyz[jb];
+                const float za = (float)s_z[ja], zb = (float)s_z[jb];
+                const V3 va = edge_vertex(s, (float)((xya & 0xffff) + ba), (float)(xya >> 16), za,
+                                          axa, f0a, f1a);
+                const uint64_t ida = vblock + r0 + e;
+                if (ida < a.vcap) *(V3 *)(a.vertices + 3 * ida) = va;
+                if (two) {
+                    const V3 vb = edge_vertex(s, (float)((xyb & 0xffff) + bb), (float)(xyb >> 16),
+                                              zb, axb, f0b, f1b);
+                    const uint64_t idb = vblock + r0 + e2;
+                    if (idb < a.vcap) *(V3 *)(a.vertices + 3 * idb) = vb;
                 }
-                const uint64_t id = vblock + r0 + e;
-                if (id < a.vcap) *(V3 *)(a.vertices + 3 * id) = V3{px, py, pz};
             }
             __syncthreads();
         }
