@@ -174,14 +174,6 @@ __device__ __forceinline__ Cross row_cross(uint64_t cur, uint64_t nxt, uint64_t 
     return c;
 }
 
-__device__ __forceinline__ uint32_t cube_at(uint64_t a, uint64_t sa, uint64_t b, uint64_t sb,
-                                            uint64_t c, uint64_t sc, uint64_t d, uint64_t sd,
-                                            uint32_t x) {
-    return bit_at(a, x) | (bit_at(sa, x) << 1) | (bit_at(sb, x) << 2) | (bit_at(b, x) << 3) |
-           (bit_at(c, x) << 4) | (bit_at(sc, x) << 5) | (bit_at(sd, x) << 6) |
-           (bit_at(d, x) << 7);
-}
-
 // a cell's corners all agree iff a == sa == b == sb == c == sc == d == sd at its bit
 __device__ __forceinline__ uint64_t active_cells(uint64_t a, uint64_t sa, uint64_t b, uint64_t sb,
                                                  uint64_t c, uint64_t sc, uint64_t d, uint64_t sd,
@@ -255,10 +247,23 @@ signs_kernel(const float *__restrict__ field, uint64_t *__restrict__ signs, int 
 }
 
 // Fills the calling wave's LDS list with the active cells of entries
-// [r0, r0 + kListCap) of its canonical order (lane | x << 6).
+// [r0, r0 + kListCap) of its canonical order (lane | x << 6); atot: the
+// wave's active cells.
 __device__ __forceinline__ void fill_list(uint16_t *list, uint64_t act, uint32_t nact, uint32_t aex,
-                                          uint32_t r0, uint32_t lane) {
-    if (nact && aex < r0 + kListCap && aex + nact > r0) {
+                                          uint32_t atot, uint32_t r0, uint32_t lane) {
+    if (atot <= kListCap) {
+        // one round holds the wave's cells: no window tests, 32-bit halves
+        uint32_t i = aex;
+#pragma unroll
+        for (uint32_t h = 0; h < 2; ++h) {
+            uint32_t m = (uint32_t)(act >> (32 * h));
+            while (m) {
+                const uint32_t x = (uint32_t)__builtin_ctz(m) + 32 * h;
+                m &= m - 1;
+                list[i++] = (uint16_t)(lane | (x << 6));
+            }
+        }
+    } else if (nact && aex < r0 + kListCap && aex + nact > r0) {
         uint64_t m = act;
         uint32_t i = aex;
         while (m) {
@@ -274,42 +279,70 @@ __device__ __forceinline__ void fill_list(uint16_t *list, uint64_t act, uint32_t
 // ---------------------------------------------------------------------------
 // count
 // ---------------------------------------------------------------------------
+// Triangles the case table gives the active cells of one word, from the
+// corner words k0..k7 (bit x of kc: corner c of cell x is inside, f < iso;
+// corners 0 (x,y,z) 1 (x+1,y,z) 2 (x+1,y+1,z) 3 (x,y+1,z) 4 (x,y,z+1)
+// 5 (x+1,y,z+1) 6 (x+1,y+1,z+1) 7 (x,y+1,z+1), the case table's numbering,
+// bit c of the case index).  For every one of the table's 255 cases
+// with an outside corner, a cell with inside-corner set S has
+//   |S| - 2 (faces of the cube with all four corners in S)
+//       - 4 [S = the cube minus both ends of a body diagonal]
+// triangles: each crossing edge is a vertex of exactly one polygon of the
+// case and a polygon of k vertices is cut into k - 2 triangles, so the count
+// is (3|S| - 2 edges in S) - 2 polygons; the polygons are as many as the
+// Euler characteristic |S| - edges + faces of S, except in the four
+// body-diagonal cases, which have two (characteristic 0).  The identity is
+// checked case by case against the table by the tests.  So a word's count is
+// a weighted sum of popcounts over its 64 cells, with no per-cell lookup.
+__device__ __forceinline__ uint32_t word_triangles(uint64_t k0, uint64_t k1, uint64_t k2,
+                                                   uint64_t k3, uint64_t k4, uint64_t k5,
+                                                   uint64_t k6, uint64_t k7, uint64_t act) {
+    const uint32_t v = popc64(k0 & act) + popc64(k1 & act) + popc64(k2 & act) +
+                       popc64(k3 & act) + popc64(k4 & act) + popc64(k5 & act) +
+                       popc64(k6 & act) + popc64(k7 & act);
+    const uint64_t p01 = k0 & k1, p23 = k2 & k3, p45 = k4 & k5, p67 = k6 & k7;
+    const uint64_t p12 = k1 & k2, p56 = k5 & k6, p03 = k0 & k3, p47 = k4 & k7;
+    const uint32_t f = popc64(p01 & p23 & act) + popc64(p45 & p67 & act) +
+                       popc64(p01 & p45 & act) + popc64(p23 & p67 & act) +
+                       popc64(p12 & p56 & act) + popc64(p03 & p47 & act);
+    // body diagonals (0,6) (1,7) (2,4) (3,5): both ends outside, the rest inside
+    const uint32_t d = popc64(~(k0 | k6) & p12 & p45 & k3 & k7 & act) +
+                       popc64(~(k1 | k7) & p03 & p56 & k2 & k4 & act) +
+                       popc64(~(k2 | k4) & p01 & p67 & k3 & k5 & act) +
+                       popc64(~(k3 | k5) & p12 & p67 & k0 & k4 & act);
+    return v - 2 * f - 4 * d;
+}
+
 struct CountArgs {
     const uint64_t *signs;  // sign buffer base
     uint32_t *twave;
     uint64_t *bsum;
-    const uint8_t *nverts;
 };
 
 template <class Src>
 __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a, Queue q) {
-    __shared__ uint8_t s_ntri[256];
-    __shared__ uint16_t s_list[4][kListCap];
     __shared__ uint32_t s_sum[4][2];
-    s_ntri[threadIdx.x] = a.nverts[threadIdx.x] / 3;
-    __syncthreads();
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for_each_block(q, [&](uint32_t gb) {
         const Seg s = src.get(src.seg_of(gb));
         const Grid &g = s.g;
         const uint64_t *signs = a.signs + s.sign_off;
         const uint64_t k = (uint64_t)(gb - s.block_off) * kBlock + tid;
-        uint64_t wa = 0, sa = 0, wb = 0, sb = 0, wc = 0, sc = 0, wd = 0, sd = 0, act = 0;
-        uint32_t vc = 0;
+        uint32_t vc = 0, tc = 0;
         if (k < g.words) {
             const WordPos p = word_pos(k, g);
             const bool has_y = p.y + 1 < g.ny, has_z = p.z + 1 < g.nz;
             const uint32_t kk = (uint32_t)k, W = (uint32_t)g.W, RW = (uint32_t)g.ny * W;
             const bool hw = p.w + 1 < g.W;
-            wa = signs[kk];
+            const uint64_t wa = signs[kk];
             const uint64_t an = ldw(signs, kk + 1, hw);
-            wb = ldw(signs, kk + W, has_y);
+            const uint64_t wb = ldw(signs, kk + W, has_y);
             const uint64_t bn = ldw(signs, kk + W + 1, has_y && hw);
-            wc = ldw(signs, kk + RW, has_z);
+            const uint64_t wc = ldw(signs, kk + RW, has_z);
             const uint64_t cn = ldw(signs, kk + RW + 1, has_z && hw);
-            sa = shift1(wa, an);
-            sb = shift1(wb, bn);
-            sc = shift1(wc, cn);
+            const uint64_t sa = shift1(wa, an);
+            const uint64_t sb = shift1(wb, bn);
+            const uint64_t sc = shift1(wc, cn);
             const uint64_t vpt = mask_below((int64_t)g.nx - 64 * p.w);
             const uint64_t ved = mask_below((int64_t)g.nx - 1 - 64 * p.w);
             const uint64_t ex = (wa ^ sa) & ved;
@@ -317,33 +350,14 @@ __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a, Que
             const uint64_t ez = has_z ? (wa ^ wc) & vpt : 0ull;
             vc = popc64(ex) + popc64(ey) + popc64(ez);
             if (has_y && has_z) {
-                wd = signs[kk + RW + W];
-                sd = shift1(wd, ldw(signs, kk + RW + W + 1, hw));
-                act = active_cells(wa, sa, wb, sb, wc, sc, wd, sd, ved);
+                const uint64_t wd = signs[kk + RW + W];
+                const uint64_t sd = shift1(wd, ldw(signs, kk + RW + W + 1, hw));
+                const uint64_t act = active_cells(wa, sa, wb, sb, wc, sc, wd, sd, ved);
+                tc = word_triangles(wa, sa, sb, wb, wc, sc, sd, wd, act);
             }
-        }
-        const uint32_t nact = popc64(act);
-        const uint32_t aincl = wave_scan_incl(nact);
-        const uint32_t atot = readlane32(aincl, 63);
-        const uint32_t aex = aincl - nact;
-        uint16_t *list = s_list[wave];
-        uint32_t tsum = 0;  // per-lane partial triangle count of the wave
-        for (uint32_t r0 = 0; r0 < atot; r0 += kListCap) {
-            fill_list(list, act, nact, aex, r0, lane);
-            const uint32_t n = min((uint32_t)kListCap, atot - r0);
-            for (uint32_t e0 = 0; e0 < n; e0 += 64) {
-                const uint32_t e = e0 + lane;
-                const uint32_t ent = e < n ? list[e] : 0u;
-                const uint32_t j = ent & 63, x = ent >> 6;
-                const uint32_t cube = cube_at(gather64(wa, j), gather64(sa, j), gather64(wb, j),
-                                              gather64(sb, j), gather64(wc, j), gather64(sc, j),
-                                              gather64(wd, j), gather64(sd, j), x);
-                if (e < n) tsum += s_ntri[cube];
-            }
-            __builtin_amdgcn_wave_barrier();
         }
         const uint32_t vw = readlane32(wave_scan_incl(vc), 63);
-        const uint32_t tw = readlane32(wave_scan_incl(tsum), 63);
+        const uint32_t tw = readlane32(wave_scan_incl(tc), 63);
         if (lane == 0) {
             s_sum[wave][0] = vw;
             s_sum[wave][1] = tw;
@@ -476,7 +490,7 @@ struct VertArgs {
 
 template <class Src>
 __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a, Queue q) {
-    __shared__ uint16_t s_list[kVertCap];
+    __shared__ uint16_t s_list[kVertCap + kBlock];  // + one discard slot per thread
     __shared__ uint64_t s_p0[kBlock];   // linear index of the word's first point
     __shared__ uint32_t s_xyz[kBlock];  // first x | y << 16
     __shared__ uint32_t s_z[kBlock];
@@ -521,7 +535,31 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a, Queue
         const uint64_t plane = (uint64_t)g.nx * g.ny;
         for (uint32_t r0 = 0; r0 < vtot; r0 += kVertCap) {
             // this lane's edges, canonical order, that fall into the round's window
-            if (vc && vex < r0 + kVertCap && vex + vc > r0) {
+            if (vtot <= kVertCap) {
+                // one round holds the block: no window tests, 32-bit halves;
+                // an edge a point lacks goes to the thread's discard slot
+                uint16_t *dump = s_list + kVertCap + tid;
+                uint32_t i = vex;
+#pragma unroll
+                for (uint32_t h = 0; h < 2; ++h) {
+                    const uint32_t hx = (uint32_t)(ex >> (32 * h));
+                    const uint32_t hy = (uint32_t)(ey >> (32 * h));
+                    const uint32_t hz = (uint32_t)(ez >> (32 * h));
+                    uint32_t pts = hx | hy | hz;
+                    while (pts) {
+                        const uint32_t b = (uint32_t)__builtin_ctz(pts);
+                        pts &= pts - 1;
+                        const uint32_t ent = (tid << 8) | ((b + 32 * h) << 2);
+                        const uint32_t bx = (hx >> b) & 1u, by = (hy >> b) & 1u, bz = (hz >> b) & 1u;
+                        *(bx ? s_list + i : dump) = (uint16_t)ent;
+                        i += bx;
+                        *(by ? s_list + i : dump) = (uint16_t)(ent | 1u);
+                        i += by;
+                        *(bz ? s_list + i : dump) = (uint16_t)(ent | 2u);
+                        i += bz;
+                    }
+                }
+            } else if (vc && vex < r0 + kVertCap && vex + vc > r0) {
                 uint64_t pts = ex | ey | ez;
                 uint32_t i = vex;
                 while (pts) {
@@ -667,7 +705,7 @@ __global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q
         uint64_t *pk = s_pk[wave];
         uint16_t *own = s_own[wave];
         for (uint32_t r0 = 0; r0 < atot; r0 += kListCap) {
-            fill_list(list, act, nact, aex, r0, lane);
+            fill_list(list, act, nact, aex, atot, r0, lane);
             const uint32_t n = min((uint32_t)kListCap, atot - r0);
             for (uint32_t e0 = 0; e0 < n; e0 += 64) {
                 const uint32_t e = e0 + lane;
@@ -801,7 +839,7 @@ int enqueue_extract(const gc_extract_args *a, hipStream_t stream) {
     const dim3 blocks((unsigned)L.nb);
     const Queue none = {nullptr, 0};
     hipLaunchKernelGGL(count_kernel<OneGrid>, blocks, dim3(kBlock), 0, stream, one,
-                       CountArgs{signs, twave, bsum, tabs->num_verts}, none);
+                       CountArgs{signs, twave, bsum}, none);
     GC_CHECK_LAUNCH();
     hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, stream, bsum, L.nb, ctl,
                        a->d_counts, (const Seg *)nullptr, 1u, res);
@@ -838,7 +876,7 @@ int enqueue_extract_batch(const BatchExtract &b, int phases, hipStream_t stream)
     if (phases & EXTRACT_COUNT) {
         const unsigned wg = (unsigned)std::max(1, std::min<int>(b.wg_count, (int)wmax));
         hipLaunchKernelGGL(count_kernel<ManyGrids>, dim3(wg), dim3(kBlock), 0, stream, src,
-                           CountArgs{b.signs, b.twave, b.bsum, tabs->num_verts},
+                           CountArgs{b.signs, b.twave, b.bsum},
                            Queue{b.queues + 0, b.nb});
         GC_CHECK_LAUNCH();
         hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, stream, b.bsum,

@@ -108,3 +108,34 @@ def test_inside_corners_lie_on_the_normal_side(tables):
                 assert side > 0, "case %d" % c
             else:
                 assert side < 0, "case %d" % c
+
+
+FACES = [(0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)]
+BODY_DIAGONALS = [(0, 6), (1, 7), (2, 4), (3, 5)]
+
+
+def count_terms(c):
+    """|S|, faces with all corners in S, [S = cube minus a body diagonal's ends]."""
+    inside = [(c >> k) & 1 for k in range(8)]
+    v = sum(inside)
+    f = sum(all(inside[k] for k in face) for face in FACES)
+    d = sum(1 for i, j in BODY_DIAGONALS if v == 6 and not inside[i] and not inside[j])
+    return v, f, d
+
+
+def test_triangle_count_identity(tables):
+    """The extraction's count stage totals triangles per 64-cell word as
+    |S| - 2F - 4D over popcounts (no per-cell table lookup); this holds only
+    because every case with an outside corner satisfies it."""
+    tri, nv, em = tables
+    without_diagonal = 0
+    for c in range(255):
+        v, f, d = count_terms(c)
+        assert nv[c] // 3 == v - 2 * f - 4 * d, "case %d" % c
+        # the same identity through crossing edges and Euler characteristic
+        edges_in = sum(1 for a, b in EDGE if (c >> a) & 1 and (c >> b) & 1)
+        polygons = (v - edges_in + f) + (2 if d else 0)
+        assert nv[c] // 3 == bin(int(em[c])).count("1") - 2 * polygons
+        without_diagonal += nv[c] // 3 != v - 2 * f
+    # the body-diagonal term matters in exactly its four cases
+    assert without_diagonal == 4
