@@ -504,7 +504,8 @@ int batch_create(const gc_lattice_params *grids, int n, const gc_batch_options *
     };
     const Part p_consts = part(n * sizeof(FieldConsts)), p_segs = part(n * sizeof(Seg)),
                p_b2s = part(nb * 4), p_res = part(n * sizeof(SegResult)),
-               p_base = part(nb * kBlock * 4), p_twave = part(nb * 16), p_bsum = part(nb * 16),
+               p_base = part(nb * kBaseBytesPerBlock),
+               p_twave = part(nb * kTwaveBytesPerBlock), p_bsum = part(nb * 16),
                p_ctl = part(64), p_q = part(kQueues * 4), p_fields = part(fo * 4),
                p_signs = part(so * 8);
     hipError_t he = hipMalloc(&b->arena, off);

@@ -44,8 +44,8 @@
 namespace gc {
 namespace {
 
-constexpr int kListCap = 512;  // per-wave LDS list entries per round
 constexpr int kVertCap = 4096; // per-block LDS edge-list entries per round
+constexpr int kEdgeStep = 4;   // edges per thread and step of the vertex emission
 
 struct ExtractLayout {
     uint64_t signs_off, base_off, twave_off, bsum_off, ctl_off, res_off, bytes;
@@ -62,9 +62,9 @@ ExtractLayout layout(int nx, int ny, int nz) {
     L.signs_off = off;
     off = align256(off + L.words * 8);
     L.base_off = off;
-    off = align256(off + L.nb * kBlock * 4);
+    off = align256(off + L.nb * kBaseBytesPerBlock);
     L.twave_off = off;
-    off = align256(off + L.nb * 4 * 4);
+    off = align256(off + L.nb * kTwaveBytesPerBlock);
     L.bsum_off = off;
     off = align256(off + L.nb * 16);
     L.ctl_off = off;  // [0] n_vertices [1] n_triangles [2] vertex numbers overflow 32 bits
@@ -94,10 +94,6 @@ __device__ __forceinline__ uint32_t lane_id() {
 
 __device__ __forceinline__ uint32_t gather32(uint32_t v, uint32_t src) {
     return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
-}
-__device__ __forceinline__ uint64_t gather64(uint64_t v, uint32_t src) {
-    uint32_t lo = gather32((uint32_t)v, src), hi = gather32((uint32_t)(v >> 32), src);
-    return ((uint64_t)hi << 32) | lo;
 }
 __device__ __forceinline__ uint32_t readlane32(uint32_t v, uint32_t l) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
@@ -246,36 +242,6 @@ signs_kernel(const float *__restrict__ field, uint64_t *__restrict__ signs, int 
     }
 }
 
-// Fills the calling wave's LDS list with the active cells of entries
-// [r0, r0 + kListCap) of its canonical order (lane | x << 6); atot: the
-// wave's active cells.
-__device__ __forceinline__ void fill_list(uint16_t *list, uint64_t act, uint32_t nact, uint32_t aex,
-                                          uint32_t atot, uint32_t r0, uint32_t lane) {
-    if (atot <= kListCap) {
-        // one round holds the wave's cells: no window tests, 32-bit halves
-        uint32_t i = aex;
-#pragma unroll
-        for (uint32_t h = 0; h < 2; ++h) {
-            uint32_t m = (uint32_t)(act >> (32 * h));
-            while (m) {
-                const uint32_t x = (uint32_t)__builtin_ctz(m) + 32 * h;
-                m &= m - 1;
-                list[i++] = (uint16_t)(lane | (x << 6));
-            }
-        }
-    } else if (nact && aex < r0 + kListCap && aex + nact > r0) {
-        uint64_t m = act;
-        uint32_t i = aex;
-        while (m) {
-            const uint32_t x = (uint32_t)__builtin_ctzll(m);
-            m &= m - 1;
-            if (i >= r0 && i < r0 + kListCap) list[i - r0] = (uint16_t)(lane | (x << 6));
-            ++i;
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-}
-
 // ---------------------------------------------------------------------------
 // count
 // ---------------------------------------------------------------------------
@@ -357,11 +323,14 @@ __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a, Que
             }
         }
         const uint32_t vw = readlane32(wave_scan_incl(vc), 63);
-        const uint32_t tw = readlane32(wave_scan_incl(tc), 63);
+        const uint32_t tincl = wave_scan_incl(tc);
+        const uint32_t tlo = readlane32(tincl, 31), tw = readlane32(tincl, 63);
         if (lane == 0) {
             s_sum[wave][0] = vw;
             s_sum[wave][1] = tw;
-            a.twave[4 * (uint64_t)gb + wave] = tw;
+            // triangle counts of the wave's two groups of 32 words
+            a.twave[kGroups * (uint64_t)gb + 2 * wave] = tlo;
+            a.twave[kGroups * (uint64_t)gb + 2 * wave + 1] = tw - tlo;
         }
         __syncthreads();
         if (tid == 0) {
@@ -388,11 +357,22 @@ scan_blocks_kernel(uint64_t *bsum, uint64_t nb, uint64_t *ctl, uint64_t *d_count
     ulonglong2 *io = (ulonglong2 *)bsum;
     const uint64_t per = (nb + 1023) / 1024;
     const uint64_t b0 = min(nb, (uint64_t)t * per), b1 = min(nb, b0 + per);
+    // entries are read kScanRun at a time, all loads of a run in flight; the
+    // first run stays in registers for the write-back
+    constexpr int kScanRun = 8;
+    ulonglong2 keep[kScanRun];
     uint64_t sv = 0, st = 0;
-    for (uint64_t i = b0; i < b1; ++i) {
-        const ulonglong2 q = io[i];
-        sv += q.x;
-        st += q.y;
+    for (uint64_t i0 = b0; i0 < b1; i0 += kScanRun) {
+        ulonglong2 q[kScanRun];
+#pragma unroll
+        for (int u = 0; u < kScanRun; ++u)
+            q[u] = i0 + u < b1 ? io[i0 + u] : make_ulonglong2(0ull, 0ull);
+#pragma unroll
+        for (int u = 0; u < kScanRun; ++u) {
+            sv += q[u].x;
+            st += q[u].y;
+            if (i0 == b0) keep[u] = q[u];
+        }
     }
     uint64_t iv = sv, it = st;  // inclusive scan within the wave
 #pragma unroll
@@ -417,11 +397,18 @@ scan_blocks_kernel(uint64_t *bsum, uint64_t nb, uint64_t *ctl, uint64_t *d_count
         carry_v += sw_v[w];
         carry_t += sw_t[w];
     }
-    for (uint64_t i = b0; i < b1; ++i) {
-        const ulonglong2 q = io[i];
-        io[i] = make_ulonglong2(rv, rt);
-        rv += q.x;
-        rt += q.y;
+    for (uint64_t i0 = b0; i0 < b1; i0 += kScanRun) {
+        ulonglong2 q[kScanRun];
+#pragma unroll
+        for (int u = 0; u < kScanRun; ++u)
+            q[u] = i0 == b0 ? keep[u]
+                            : (i0 + u < b1 ? io[i0 + u] : make_ulonglong2(0ull, 0ull));
+#pragma unroll
+        for (int u = 0; u < kScanRun; ++u) {
+            if (i0 + u < b1) io[i0 + u] = make_ulonglong2(rv, rt);
+            rv += q[u].x;
+            rt += q[u].y;
+        }
     }
     __syncthreads();
     // every thread holds the totals; the scanned sums are visible to the workgroup
@@ -502,6 +489,10 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a, Queue
         const Grid &g = s.g;
         const uint64_t *signs = a.signs + s.sign_off;
         const uint64_t k = (uint64_t)(gb - s.block_off) * kBlock + tid;
+        // block base and grid range first: their loads overlap the sign loads
+        const uint64_t vblock = a.bsum[2 * (uint64_t)gb];
+        const uint64_t v_off = a.res[si].v_off;
+        const uint32_t ovf = a.res[si].ovf;
         uint64_t ex = 0, ey = 0, ez = 0;
         if (k < g.words) {
             const WordPos p = word_pos(k, g);
@@ -527,10 +518,14 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a, Queue
             vtot += s_wsum[i];
         }
         const uint32_t vex = wbase + incl - vc;  // block-local first vertex of this word
-        const uint64_t vblock = a.bsum[2 * (uint64_t)gb];
-        const SegResult &r = a.res[si];
-        if (k < g.words) a.base[(uint64_t)gb * kBlock + tid] = (uint32_t)(vblock - r.v_off + vex);
-        if (r.ovf) return;  // vertex numbers do not fit 32 bits: nothing is written
+        if (k < g.words) {
+            // first vertex numbers of the word's two 32-point halves
+            const uint32_t v0 = (uint32_t)(vblock - v_off + vex);
+            const uint32_t lo = popc64(ex & 0xffffffffull) + popc64(ey & 0xffffffffull) +
+                                popc64(ez & 0xffffffffull);
+            *(uint2 *)(a.base + 2 * ((uint64_t)gb * kBlock + tid)) = make_uint2(v0, v0 + lo);
+        }
+        if (ovf) return;  // vertex numbers do not fit 32 bits: nothing is written
         const float *field = a.field + s.field_off;
         const uint64_t plane = (uint64_t)g.nx * g.ny;
         for (uint32_t r0 = 0; r0 < vtot; r0 += kVertCap) {
@@ -578,30 +573,30 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a, Queue
             }
             __syncthreads();
             const uint32_t n = min((uint32_t)kVertCap, vtot - r0);
-            // two edges per thread and step, so that four field loads are in
-            // flight before the first interpolation
-            for (uint32_t e = tid; e < n; e += 2 * kBlock) {
-                const uint32_t e2 = e + kBlock;
-                const bool two = e2 < n;
-                const uint32_t ea = s_list[e], eb = s_list[two ? e2 : e];
-                const uint32_t ja = ea >> 8, ba = (ea >> 2) & 63, axa = ea & 3;
-                const uint32_t jb = eb >> 8, bb = (eb >> 2) & 63, axb = eb & 3;
-                const uint64_t pa = s_p0[ja] + ba, pb = s_p0[jb] + bb;
-                const uint64_t sa = axa == 0 ? 1ull : (axa == 1 ? (uint64_t)g.nx : plane);
-                const uint64_t sb = axb == 0 ? 1ull : (axb == 1 ? (uint64_t)g.nx : plane);
-                const float f0a = field[pa], f1a = field[pa + sa];
-                const float f0b = field[pb], f1b = field[pb + sb];
-                const uint32_t xya = s_xyz[ja], xyb = s_xyz[jb];
-                const float za = (float)s_z[ja], zb = (float)s_z[jb];
-                const V3 va = edge_vertex(s, (float)((xya & 0xffff) + ba), (float)(xya >> 16), za,
-                                          axa, f0a, f1a);
-                const uint64_t ida = vblock + r0 + e;
-                if (ida < a.vcap) *(V3 *)(a.vertices + 3 * ida) = va;
-                if (two) {
-                    const V3 vb = edge_vertex(s, (float)((xyb & 0xffff) + bb), (float)(xyb >> 16),
-                                              zb, axb, f0b, f1b);
-                    const uint64_t idb = vblock + r0 + e2;
-                    if (idb < a.vcap) *(V3 *)(a.vertices + 3 * idb) = vb;
+            // kEdgeStep edges per thread and step, so that all their field
+            // loads are in flight before the first interpolation
+            for (uint32_t e0 = tid; e0 < n; e0 += kEdgeStep * kBlock) {
+                uint32_t ent[kEdgeStep];
+                float f0[kEdgeStep], f1[kEdgeStep];
+#pragma unroll
+                for (int q = 0; q < kEdgeStep; ++q) {
+                    const uint32_t e = e0 + q * kBlock;
+                    ent[q] = s_list[e < n ? e : e0];
+                    const uint32_t j = ent[q] >> 8, b = (ent[q] >> 2) & 63, ax = ent[q] & 3;
+                    const uint64_t pt = s_p0[j] + b;
+                    const uint64_t st = ax == 0 ? 1ull : (ax == 1 ? (uint64_t)g.nx : plane);
+                    f0[q] = field[pt];
+                    f1[q] = field[pt + st];
+                }
+#pragma unroll
+                for (int q = 0; q < kEdgeStep; ++q) {
+                    const uint32_t e = e0 + q * kBlock;
+                    const uint32_t j = ent[q] >> 8, b = (ent[q] >> 2) & 63, ax = ent[q] & 3;
+                    const uint32_t xy = s_xyz[j];
+                    const V3 v = edge_vertex(s, (float)((xy & 0xffff) + b), (float)(xy >> 16),
+                                             (float)s_z[j], ax, f0[q], f1[q]);
+                    const uint64_t id = vblock + r0 + e;
+                    if (e < n && id < a.vcap) *(V3 *)(a.vertices + 3 * id) = v;
                 }
             }
             __syncthreads();
@@ -612,10 +607,78 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a, Queue
 // ---------------------------------------------------------------------------
 // triangles
 // ---------------------------------------------------------------------------
+// One thread per half word: 32 points of a row, bit b of half h of word w is
+// point 64w + 32h + b, and the sign buffer is read as 32-bit halves.  A
+// triangle block is half of a count block: 128 words, 4 waves of 32 words,
+// each wave one of the count stage's triangle groups.  32-bit halves halve
+// the gathers and the bit work per cell and the per-thread state.
+constexpr int kTriThreads = kBlock;
+constexpr int kTriWords = kTriThreads / 2;  // words per triangle block
+constexpr int kCellCap = 256;  // per-wave LDS cell-list entries per round
+
+__device__ __forceinline__ uint32_t sh1(uint32_t cur, uint32_t next) {
+    return (cur >> 1) | (next << 31);
+}
+// bits [0, n) of a 32-bit half
+__device__ __forceinline__ uint32_t below32(int64_t n) {
+    return n <= 0 ? 0u : (n >= 32 ? ~0u : (1u << n) - 1u);
+}
+__device__ __forceinline__ uint32_t ldh(const uint32_t *s, uint32_t idx, bool ok) {
+    return ok ? s[idx] : 0u;
+}
+__device__ __forceinline__ uint32_t bit32(uint32_t w, uint32_t b) { return (w >> b) & 1u; }
+
+struct Cross32 {
+    uint32_t ex, ey, ez, sx, sy;
+};
+
+// Crossing bits of the row half with signs (cur, nxt = next half), +y
+// neighbour (py, pyn) and +z neighbour pz; sx/sy: the same for point b+1.
+__device__ __forceinline__ Cross32 row_cross32(uint32_t cur, uint32_t nxt, uint32_t py,
+                                               uint32_t pyn, uint32_t pz, bool has_y, bool has_z,
+                                               uint32_t ved, uint32_t vpt, bool ved_n,
+                                               bool vpt_n) {
+    Cross32 c;
+    c.ex = (cur ^ sh1(cur, nxt)) & ved;
+    c.ey = has_y ? (cur ^ py) & vpt : 0u;
+    c.ez = has_z ? (cur ^ pz) & vpt : 0u;
+    const uint32_t exn = ved_n ? ((nxt ^ (nxt >> 1)) & 1u) : 0u;
+    const uint32_t eyn = (has_y && vpt_n) ? ((nxt ^ pyn) & 1u) : 0u;
+    c.sx = sh1(c.ex, exn);
+    c.sy = sh1(c.ey, eyn);
+    return c;
+}
+
+// Fills the calling wave's LDS list with its active cells of entries
+// [r0, r0 + kCellCap) of the canonical order (lane | x << 6).
+__device__ __forceinline__ void fill_cells(uint16_t *list, uint32_t act, uint32_t nact,
+                                           uint32_t aex, uint32_t atot, uint32_t r0,
+                                           uint32_t lane) {
+    if (atot <= kCellCap) {
+        uint32_t i = aex;
+        uint32_t m = act;
+        while (m) {
+            const uint32_t x = (uint32_t)__builtin_ctz(m);
+            m &= m - 1;
+            list[i++] = (uint16_t)(lane | (x << 6));
+        }
+    } else if (nact && aex < r0 + kCellCap && aex + nact > r0) {
+        uint32_t m = act;
+        uint32_t i = aex;
+        while (m) {
+            const uint32_t x = (uint32_t)__builtin_ctz(m);
+            m &= m - 1;
+            if (i >= r0 && i < r0 + kCellCap) list[i - r0] = (uint16_t)(lane | (x << 6));
+            ++i;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
 struct TriArgs {
     const uint64_t *signs;  // sign buffer base
-    const uint32_t *base;
-    const uint32_t *twave;
+    const uint32_t *base;   // first vertex number of each half word
+    const uint32_t *twave;  // triangles of each group of 32 words
     const uint64_t *bsum;
     const uint64_t *tri_packed;
     const SegResult *res;
@@ -624,112 +687,117 @@ struct TriArgs {
 };
 
 template <class Src>
-__global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q) {
+__global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Queue q) {
+    constexpr int kWaves = kTriThreads / 64;
     __shared__ uint64_t s_tri[256];
-    __shared__ uint16_t s_list[4][kListCap];
-    __shared__ uint32_t s_vid[4][12][64];
-    __shared__ uint64_t s_pk[4][64];      // packed triangle list of each lane's cell
-    __shared__ uint16_t s_own[4][5 * 64]; // triangle -> lane | (triangle of the cell) << 8
-    s_tri[threadIdx.x] = a.tri_packed[threadIdx.x];
+    __shared__ uint16_t s_list[kWaves][kCellCap];
+    __shared__ uint32_t s_vid[kWaves][12 * 64];  // id k of lane l at k*64 + (l ^ k)
+    __shared__ uint64_t s_pk[kWaves][64];       // packed triangle list of each lane's cell
+    __shared__ uint16_t s_own[kWaves][5 * 64];  // triangle -> lane | (triangle of the cell) << 8
+    if (threadIdx.x < 256) s_tri[threadIdx.x] = a.tri_packed[threadIdx.x];
     __syncthreads();
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for_each_block(q, [&](uint32_t gb) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = tid & 1;
+    for_each_block(q, [&](uint32_t tb) {
+        const uint32_t gb = tb >> 1, part = tb & 1;  // count block and its half
         const uint32_t si = src.seg_of(gb);
         if (a.res[si].ovf) return;  // vertex numbers do not fit 32 bits
         const Seg s = src.get(si);
         const Grid &g = s.g;
-        const uint64_t *signs = a.signs + s.sign_off;
-        const uint64_t k = (uint64_t)(gb - s.block_off) * kBlock + tid;
-        const uint32_t *base = a.base + (uint64_t)s.block_off * kBlock;  // by grid word
-        // ---- per word (owner lane): the sign word of row (y, z) and the
-        // crossing words of the four rows (y..y+1, z..z+1) touching its cells;
-        // a cell's cube index and edge vertex numbers follow from these bits
-        // (sign of a neighbour corner = own sign ^ crossing of the edge between)
-        uint64_t wa = 0, act = 0;
-        uint64_t ex0 = 0, ey0 = 0, ez0 = 0, ex1 = 0, ey1 = 0, ez1 = 0;
-        uint64_t ex2 = 0, ey2 = 0, ez2 = 0, ex3 = 0, ey3 = 0, ez3 = 0;
-        uint32_t carry = 0;  // crossing bits of point 64 (next word): x0 y0 x1 y1 x2
+        const uint32_t *signs = (const uint32_t *)(a.signs + s.sign_off);
+        const uint64_t k = (uint64_t)(gb - s.block_off) * kBlock + part * kTriWords + (tid >> 1);
+        const uint32_t *base = a.base + (uint64_t)s.block_off * kBlock * 2;  // by grid half word
+        // ---- per half word (owner lane): the sign half of row (y, z) and the
+        // crossing halves of the four rows (y..y+1, z..z+1) touching its
+        // cells; a cell's cube index and edge vertex numbers follow from these
+        // bits (sign of a neighbour corner = own sign ^ crossing of the edge
+        // between)
+        uint32_t wa = 0, act = 0;
+        uint32_t ex0 = 0, ey0 = 0, ez0 = 0, ex1 = 0, ey1 = 0, ez1 = 0;
+        uint32_t ex2 = 0, ey2 = 0, ez2 = 0, ex3 = 0, ey3 = 0, ez3 = 0;
+        uint32_t carry = 0;  // crossing bits of point 32 (next half): x0 y0 x1 y1 x2
         uint32_t vb0 = 0, vb1 = 0, vb2 = 0, vb3 = 0;
         if (k < g.words) {
             const WordPos p = word_pos(k, g);
-            if (p.y + 1 < g.ny && p.z + 1 < g.nz) {
+            const int xh = 64 * p.w + 32 * (int)h;  // first point of the half
+            const uint32_t ved = below32((int64_t)g.nx - 1 - xh);
+            if (p.y + 1 < g.ny && p.z + 1 < g.nz && ved) {
                 const bool has_y2 = p.y + 2 < g.ny, has_z2 = p.z + 2 < g.nz;
-                const int w = p.w;
-                const uint32_t kk = (uint32_t)k, W = (uint32_t)g.W, RW = (uint32_t)g.ny * W;
-                const bool hw = w + 1 < g.W;
-                wa = signs[kk];
-                const uint64_t na = ldw(signs, kk + 1, hw);
-                const uint64_t wb = signs[kk + W];
-                const uint64_t nb = ldw(signs, kk + W + 1, hw);
-                const uint64_t wc = signs[kk + RW];
-                const uint64_t nc = ldw(signs, kk + RW + 1, hw);
-                const uint64_t wd = signs[kk + RW + W];
-                const uint64_t nd = ldw(signs, kk + RW + W + 1, hw);
-                const uint64_t ved = mask_below((int64_t)g.nx - 1 - 64 * w);
-                act = active_cells(wa, shift1(wa, na), wb, shift1(wb, nb), wc, shift1(wc, nc), wd,
-                                   shift1(wd, nd), ved);
+                const uint32_t W2 = 2u * (uint32_t)g.W, RW2 = (uint32_t)g.ny * W2;
+                const uint32_t i0 = 2u * (uint32_t)k + h;
+                const bool hn = h == 0 || p.w + 1 < g.W;  // the row has a next half
+                wa = signs[i0];
+                const uint32_t na = ldh(signs, i0 + 1, hn);
+                const uint32_t wb = signs[i0 + W2];
+                const uint32_t nb = ldh(signs, i0 + W2 + 1, hn);
+                const uint32_t wc = signs[i0 + RW2];
+                const uint32_t nc = ldh(signs, i0 + RW2 + 1, hn);
+                const uint32_t wd = signs[i0 + RW2 + W2];
+                const uint32_t nd = ldh(signs, i0 + RW2 + W2 + 1, hn);
+                const uint32_t sa = sh1(wa, na), sb = sh1(wb, nb), sc = sh1(wc, nc),
+                               sd = sh1(wd, nd);
+                act = ((wa ^ sa) | (wa ^ wb) | (wa ^ sb) | (wa ^ wc) | (wa ^ sc) | (wa ^ wd) |
+                       (wa ^ sd)) & ved;
                 if (act) {
-                    const uint64_t vpt = mask_below((int64_t)g.nx - 64 * w);
-                    const bool vpt_n = 64 * (w + 1) < g.nx, ved_n = 64 * (w + 1) < g.nx - 1;
-                    const uint64_t we = ldw(signs, kk + 2 * W, has_y2),
-                                   ne = ldw(signs, kk + 2 * W + 1, has_y2 && hw);
-                    const uint64_t wf = ldw(signs, kk + 2 * W + RW, has_y2);
-                    const uint64_t wg = ldw(signs, kk + 2 * RW, has_z2);
-                    const uint64_t wh = ldw(signs, kk + 2 * RW + W, has_z2);
-                    const Cross c0 = row_cross(wa, na, wb, nb, wc, true, true, ved, vpt, ved_n, vpt_n);
-                    const Cross c1 = row_cross(wb, nb, we, ne, wd, has_y2, true, ved, vpt, ved_n, vpt_n);
-                    const Cross c2 = row_cross(wc, nc, wd, nd, wg, true, has_z2, ved, vpt, ved_n, vpt_n);
-                    const Cross c3 = row_cross(wd, nd, wf, 0ull, wh, has_y2, has_z2, ved, vpt, ved_n, vpt_n);
+                    const uint32_t vpt = below32((int64_t)g.nx - xh);
+                    const bool vpt_n = xh + 32 < g.nx, ved_n = xh + 32 < g.nx - 1;
+                    const uint32_t we = ldh(signs, i0 + 2 * W2, has_y2),
+                                   ne = ldh(signs, i0 + 2 * W2 + 1, has_y2 && hn);
+                    const uint32_t wf = ldh(signs, i0 + 2 * W2 + RW2, has_y2);
+                    const uint32_t wg = ldh(signs, i0 + 2 * RW2, has_z2);
+                    const uint32_t wh = ldh(signs, i0 + 2 * RW2 + W2, has_z2);
+                    const Cross32 c0 = row_cross32(wa, na, wb, nb, wc, true, true, ved, vpt, ved_n, vpt_n);
+                    const Cross32 c1 = row_cross32(wb, nb, we, ne, wd, has_y2, true, ved, vpt, ved_n, vpt_n);
+                    const Cross32 c2 = row_cross32(wc, nc, wd, nd, wg, true, has_z2, ved, vpt, ved_n, vpt_n);
+                    const Cross32 c3 = row_cross32(wd, nd, wf, 0u, wh, has_y2, has_z2, ved, vpt, ved_n, vpt_n);
                     ex0 = c0.ex; ey0 = c0.ey; ez0 = c0.ez;
                     ex1 = c1.ex; ey1 = c1.ey; ez1 = c1.ez;
                     ex2 = c2.ex; ey2 = c2.ey; ez2 = c2.ez;
                     ex3 = c3.ex; ey3 = c3.ey; ez3 = c3.ez;
-                    carry = (uint32_t)(c0.sx >> 63) | ((uint32_t)(c0.sy >> 63) << 1) |
-                            ((uint32_t)(c1.sx >> 63) << 2) | ((uint32_t)(c1.sy >> 63) << 3) |
-                            ((uint32_t)(c2.sx >> 63) << 4);
-                    vb0 = base[kk];
-                    vb1 = base[kk + W];
-                    vb2 = base[kk + RW];
-                    vb3 = base[kk + RW + W];
+                    carry = (c0.sx >> 31) | ((c0.sy >> 31) << 1) | ((c1.sx >> 31) << 2) |
+                            ((c1.sy >> 31) << 3) | ((c2.sx >> 31) << 4);
+                    vb0 = base[i0];
+                    vb1 = base[i0 + W2];
+                    vb2 = base[i0 + RW2];
+                    vb3 = base[i0 + RW2 + W2];
                 }
             }
         }
-        const uint32_t nact = popc64(act);
+        const uint32_t nact = (uint32_t)__popc(act);
         const uint32_t aincl = wave_scan_incl(nact);
         const uint32_t atot = readlane32(aincl, 63);
         const uint32_t aex = aincl - nact;
         uint16_t *list = s_list[wave];
         uint64_t tnext = a.bsum[2 * (uint64_t)gb + 1];
-        for (uint32_t i = 0; i < wave; ++i) tnext += a.twave[4 * (uint64_t)gb + i];
-        uint32_t(*vid)[64] = s_vid[wave];
+        for (uint32_t i = 0; i < part * 4 + wave; ++i) tnext += a.twave[kGroups * (uint64_t)gb + i];
+        uint32_t *vid = s_vid[wave];
         uint64_t *pk = s_pk[wave];
         uint16_t *own = s_own[wave];
-        for (uint32_t r0 = 0; r0 < atot; r0 += kListCap) {
-            fill_list(list, act, nact, aex, atot, r0, lane);
-            const uint32_t n = min((uint32_t)kListCap, atot - r0);
+        for (uint32_t r0 = 0; r0 < atot; r0 += kCellCap) {
+            fill_cells(list, act, nact, aex, atot, r0, lane);
+            const uint32_t n = min((uint32_t)kCellCap, atot - r0);
             for (uint32_t e0 = 0; e0 < n; e0 += 64) {
                 const uint32_t e = e0 + lane;
                 const bool on = e < n;
                 const uint32_t ent = on ? list[e] : 0u;
                 const uint32_t j = ent & 63, x = ent >> 6;
-                const uint64_t gx0 = gather64(ex0, j), gy0 = gather64(ey0, j), gz0 = gather64(ez0, j);
-                const uint64_t gx1 = gather64(ex1, j), gy1 = gather64(ey1, j), gz1 = gather64(ez1, j);
-                const uint64_t gx2 = gather64(ex2, j), gy2 = gather64(ey2, j), gz2 = gather64(ez2, j);
-                const uint64_t gx3 = gather64(ex3, j), gy3 = gather64(ey3, j), gz3 = gather64(ez3, j);
-                const uint32_t a0 = bit_at(gather64(wa, j), x);
+                const uint32_t gx0 = gather32(ex0, j), gy0 = gather32(ey0, j), gz0 = gather32(ez0, j);
+                const uint32_t gx1 = gather32(ex1, j), gy1 = gather32(ey1, j), gz1 = gather32(ez1, j);
+                const uint32_t gx2 = gather32(ex2, j), gy2 = gather32(ey2, j), gz2 = gather32(ez2, j);
+                const uint32_t gx3 = gather32(ex3, j), gy3 = gather32(ey3, j), gz3 = gather32(ez3, j);
+                const uint32_t a0 = bit32(gather32(wa, j), x);
                 const uint32_t cy = gather32(carry, j);
-                // bits of point x+1 (the next word's point 0 when x = 63)
-                const bool last = x == 63;
+                // bits of point x+1 (the next half's point 0 when x = 31)
+                const bool last = x == 31;
                 const uint32_t xn = last ? 0u : x + 1;
-                const uint32_t sx0 = last ? (cy & 1u) : bit_at(gx0, xn);
-                const uint32_t sy0 = last ? ((cy >> 1) & 1u) : bit_at(gy0, xn);
-                const uint32_t sx1 = last ? ((cy >> 2) & 1u) : bit_at(gx1, xn);
-                const uint32_t sy1 = last ? ((cy >> 3) & 1u) : bit_at(gy1, xn);
-                const uint32_t sx2 = last ? ((cy >> 4) & 1u) : bit_at(gx2, xn);
-                const uint32_t x0 = bit_at(gx0, x), y0 = bit_at(gy0, x), z0 = bit_at(gz0, x);
-                const uint32_t x1 = bit_at(gx1, x), y1 = bit_at(gy1, x), z1 = bit_at(gz1, x);
-                const uint32_t x2 = bit_at(gx2, x), y2 = bit_at(gy2, x), z2 = bit_at(gz2, x);
-                const uint32_t x3 = bit_at(gx3, x);
+                const uint32_t sx0 = last ? (cy & 1u) : bit32(gx0, xn);
+                const uint32_t sy0 = last ? ((cy >> 1) & 1u) : bit32(gy0, xn);
+                const uint32_t sx1 = last ? ((cy >> 2) & 1u) : bit32(gx1, xn);
+                const uint32_t sy1 = last ? ((cy >> 3) & 1u) : bit32(gy1, xn);
+                const uint32_t sx2 = last ? ((cy >> 4) & 1u) : bit32(gx2, xn);
+                const uint32_t x0 = bit32(gx0, x), y0 = bit32(gy0, x), z0 = bit32(gz0, x);
+                const uint32_t x1 = bit32(gx1, x), y1 = bit32(gy1, x), z1 = bit32(gz1, x);
+                const uint32_t x2 = bit32(gx2, x), y2 = bit32(gy2, x), z2 = bit32(gz2, x);
+                const uint32_t x3 = bit32(gx3, x);
                 // corners: 0 (x,y,z) 1 (x+1,y,z) 2 (x+1,y+1,z) 3 (x,y+1,z)
                 //          4 (x,y,z+1) 5 (x+1,y,z+1) 6 (x+1,y+1,z+1) 7 (x,y+1,z+1)
                 const uint32_t k0 = a0, k1 = a0 ^ x0, k3 = a0 ^ y0, k2 = k3 ^ x1;
@@ -740,31 +808,31 @@ __global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q
                 const uint32_t nt = on ? (uint32_t)(packed >> 60) : 0u;
                 const uint32_t tincl = wave_scan_incl(nt);
                 const uint32_t tex = tincl - nt, ttot = readlane32(tincl, 63);
-                const uint64_t below = x ? (~0ull >> (64 - x)) : 0ull;
-                const uint32_t p0 = gather32(vb0, j) + popc64(gx0 & below) + popc64(gy0 & below) +
-                                    popc64(gz0 & below);
-                const uint32_t p1 = gather32(vb1, j) + popc64(gx1 & below) + popc64(gy1 & below) +
-                                    popc64(gz1 & below);
-                const uint32_t p2 = gather32(vb2, j) + popc64(gx2 & below) + popc64(gy2 & below) +
-                                    popc64(gz2 & below);
-                const uint32_t p3 = gather32(vb3, j) + popc64(gx3 & below) + popc64(gy3 & below) +
-                                    popc64(gz3 & below);
+                const uint32_t below = (1u << x) - 1u;
+                const uint32_t p0 = gather32(vb0, j) + __popc(gx0 & below) + __popc(gy0 & below) +
+                                    __popc(gz0 & below);
+                const uint32_t p1 = gather32(vb1, j) + __popc(gx1 & below) + __popc(gy1 & below) +
+                                    __popc(gz1 & below);
+                const uint32_t p2 = gather32(vb2, j) + __popc(gx2 & below) + __popc(gy2 & below) +
+                                    __popc(gz2 & below);
+                const uint32_t p3 = gather32(vb3, j) + __popc(gx3 & below) + __popc(gy3 & below) +
+                                    __popc(gz3 & below);
                 const uint32_t q0 = p0 + x0 + y0 + z0;  // first edge of point x+1
                 const uint32_t q1 = p1 + x1 + y1 + z1;
                 const uint32_t q2 = p2 + x2 + y2 + z2;
                 if (on) {
-                    vid[0][lane] = p0;              // x-edge of (x,   y,   z)
-                    vid[1][lane] = q0 + sx0;        // y-edge of (x+1, y,   z)
-                    vid[2][lane] = p1;              // x-edge of (x,   y+1, z)
-                    vid[3][lane] = p0 + x0;         // y-edge of (x,   y,   z)
-                    vid[4][lane] = p2;              // x-edge of (x,   y,   z+1)
-                    vid[5][lane] = q2 + sx2;        // y-edge of (x+1, y,   z+1)
-                    vid[6][lane] = p3;              // x-edge of (x,   y+1, z+1)
-                    vid[7][lane] = p2 + x2;         // y-edge of (x,   y,   z+1)
-                    vid[8][lane] = p0 + x0 + y0;    // z-edge of (x,   y,   z)
-                    vid[9][lane] = q0 + sx0 + sy0;  // z-edge of (x+1, y,   z)
-                    vid[10][lane] = q1 + sx1 + sy1; // z-edge of (x+1, y+1, z)
-                    vid[11][lane] = p1 + x1 + y1;   // z-edge of (x,   y+1, z)
+                    vid[0 * 64 + (lane ^ 0)] = p0;              // x-edge of (x,   y,   z)
+                    vid[1 * 64 + (lane ^ 1)] = q0 + sx0;        // y-edge of (x+1, y,   z)
+                    vid[2 * 64 + (lane ^ 2)] = p1;              // x-edge of (x,   y+1, z)
+                    vid[3 * 64 + (lane ^ 3)] = p0 + x0;         // y-edge of (x,   y,   z)
+                    vid[4 * 64 + (lane ^ 4)] = p2;              // x-edge of (x,   y,   z+1)
+                    vid[5 * 64 + (lane ^ 5)] = q2 + sx2;        // y-edge of (x+1, y,   z+1)
+                    vid[6 * 64 + (lane ^ 6)] = p3;              // x-edge of (x,   y+1, z+1)
+                    vid[7 * 64 + (lane ^ 7)] = p2 + x2;         // y-edge of (x,   y,   z+1)
+                    vid[8 * 64 + (lane ^ 8)] = p0 + x0 + y0;    // z-edge of (x,   y,   z)
+                    vid[9 * 64 + (lane ^ 9)] = q0 + sx0 + sy0;  // z-edge of (x+1, y,   z)
+                    vid[10 * 64 + (lane ^ 10)] = q1 + sx1 + sy1; // z-edge of (x+1, y+1, z)
+                    vid[11 * 64 + (lane ^ 11)] = p1 + x1 + y1;   // z-edge of (x,   y+1, z)
                     pk[lane] = packed;
                     for (uint32_t jt = 0; jt < nt; ++jt) own[tex + jt] = (uint16_t)(lane | (jt << 8));
                 }
@@ -776,9 +844,12 @@ __global__ void __launch_bounds__(kBlock) tri_kernel(Src src, TriArgs a, Queue q
                     const uint64_t pkl = pk[l];
                     const uint64_t t = tnext + te;
                     if (t < a.tcap) {
-                        const uint32_t v0 = vid[(pkl >> (12 * jt)) & 15][l];
-                        const uint32_t v1 = vid[(pkl >> (12 * jt + 4)) & 15][l];
-                        const uint32_t v2 = vid[(pkl >> (12 * jt + 8)) & 15][l];
+                        const uint32_t e0v = (uint32_t)(pkl >> (12 * jt)) & 15u;
+                        const uint32_t e1v = (uint32_t)(pkl >> (12 * jt + 4)) & 15u;
+                        const uint32_t e2v = (uint32_t)(pkl >> (12 * jt + 8)) & 15u;
+                        const uint32_t v0 = vid[e0v * 64 + (l ^ e0v)];
+                        const uint32_t v1 = vid[e1v * 64 + (l ^ e1v)];
+                        const uint32_t v2 = vid[e2v * 64 + (l ^ e2v)];
                         *(T3 *)(a.triangles + 3 * t) = T3{v0, v1, v2};
                     }
                 }
@@ -850,7 +921,8 @@ int enqueue_extract(const gc_extract_args *a, hipStream_t stream) {
     GC_CHECK_LAUNCH();
     const TriArgs ta = {signs, base, twave, bsum, tabs->tri_packed, res, a->triangles,
                         a->triangles ? a->tri_capacity : 0};
-    hipLaunchKernelGGL(tri_kernel<OneGrid>, blocks, dim3(kBlock), 0, stream, one, ta, none);
+    hipLaunchKernelGGL(tri_kernel<OneGrid>, dim3(2 * (unsigned)L.nb), dim3(kTriThreads), 0, stream,
+                       one, ta, none);
     GC_CHECK_LAUNCH();
     return GPUCAD_OK;
 }
@@ -859,7 +931,7 @@ int enqueue_extract(const gc_extract_args *a, hipStream_t stream) {
 int extract_batch_occupancy(int *count, int *vert, int *tri) {
     GC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(count, count_kernel<ManyGrids>, kBlock, 0));
     GC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(vert, vert_kernel<ManyGrids>, kBlock, 0));
-    GC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(tri, tri_kernel<ManyGrids>, kBlock, 0));
+    GC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(tri, tri_kernel<ManyGrids>, kTriThreads, 0));
     *count = std::max(1, *count);
     *vert = std::max(1, *vert);
     *tri = std::max(1, *tri);
@@ -890,11 +962,11 @@ int enqueue_extract_batch(const BatchExtract &b, int phases, hipStream_t stream)
         hipLaunchKernelGGL(vert_kernel<ManyGrids>, dim3(wv), dim3(kBlock), 0, stream, src, va,
                            Queue{b.queues + 1, b.nb});
         GC_CHECK_LAUNCH();
-        const unsigned wt = (unsigned)std::max(1, std::min<int>(b.wg_tri, (int)wmax));
+        const unsigned wt = (unsigned)std::max(1, std::min<int>(b.wg_tri, 2 * (int)wmax));
         const TriArgs ta = {b.signs, b.base, b.twave, b.bsum, tabs->tri_packed, b.res,
                             b.triangles, b.triangles ? b.tcap : 0};
-        hipLaunchKernelGGL(tri_kernel<ManyGrids>, dim3(wt), dim3(kBlock), 0, stream, src, ta,
-                           Queue{b.queues + 2, b.nb});
+        hipLaunchKernelGGL(tri_kernel<ManyGrids>, dim3(wt), dim3(kTriThreads), 0, stream, src, ta,
+                           Queue{b.queues + 2, 2 * b.nb});
         GC_CHECK_LAUNCH();
     }
     return GPUCAD_OK;

@@ -11,6 +11,11 @@
 namespace gc {
 
 constexpr int kBlock = 256;  // words per extraction block (4 waves)
+constexpr int kGroups = 8;   // triangle-count groups (32 words each) per block
+// per block: first vertex number of each 32-point half word (uint32), and the
+// triangle count of each group (uint32)
+constexpr uint64_t kBaseBytesPerBlock = (uint64_t)kBlock * 2 * 4;
+constexpr uint64_t kTwaveBytesPerBlock = (uint64_t)kGroups * 4;
 
 // n / d for n < 2^32 through a double reciprocal and one correction step
 struct FastDiv {
@@ -70,7 +75,7 @@ struct BatchExtract {
     const Seg *segs;           // device, nseg
     const uint32_t *blk2seg;   // device, nb: grid of each global block
     uint32_t nseg, nb;
-    uint32_t *base, *twave;    // nb * kBlock, nb * 4
+    uint32_t *base, *twave;    // nb * kBlock * 2, nb * kGroups
     uint64_t *bsum;            // nb * 2
     uint64_t *ctl;             // 8 x uint64: totals, any overflow
     SegResult *res;            // device, nseg

@@ -115,8 +115,9 @@ uint64_t work_bytes(const gc_lattice_params &p, const Shape &sh) {
          round_up(sizeof(SegResult), kGranule);
     b += round_up(brick_count(p, sh.bx, sh.lz) * 8, kGranule);
     b += round_up(nb * 4, kGranule);                    // blk2seg
-    b += round_up(nb * kBlock * 4, kGranule);           // base
-    b += round_up(nb * 16, kGranule) * 2;               // twave, bsum
+    b += round_up(nb * kBaseBytesPerBlock, kGranule);   // base
+    b += round_up(nb * kTwaveBytesPerBlock, kGranule);  // twave
+    b += round_up(nb * 16, kGranule);                   // bsum
     b += round_up(round_up(words, 32) * 8, kGranule);   // sign words
     return b;
 }
@@ -302,8 +303,8 @@ struct Server {
         const uint64_t o_segs = take(n * sizeof(Seg));
         const uint64_t o_b2s = take(nb * 4);
         const uint64_t o_res = take(n * sizeof(SegResult));
-        const uint64_t o_base = take(nb * kBlock * 4);
-        const uint64_t o_twave = take(nb * 16);
+        const uint64_t o_base = take(nb * kBaseBytesPerBlock);
+        const uint64_t o_twave = take(nb * kTwaveBytesPerBlock);
         const uint64_t o_bsum = take(nb * 16);
         const uint64_t o_ctl = take(64);
         const uint64_t o_q = take(kQueues * 4);
