@@ -66,6 +66,8 @@ def parse():
                     help="also run a 512 MiB device copy (counter calibration)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched form")
     ap.add_argument("--no-serve", action="store_true", help="skip the served form")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 128^3 plain gyroid and 1024^3 bent diamond workloads")
     ap.add_argument("--batch-runs", type=int, default=10)
     ap.add_argument("--serve-requests", type=int, default=1000)
     return ap.parse_args()
@@ -109,6 +111,46 @@ class Pipeline:
 
     def extract_pass(self):
         gc.check(gc.lib.gpucad_extract(gc.C.byref(self.args), None, self.h))
+
+
+def bench_config(p, name, steps, warmup):
+    """Field + indexed extraction of one more single-grid workload (same
+    timing as the headline step: HIP events around each pass)."""
+    stream = torch.cuda.Stream()
+    pipe = Pipeline(p, stream)
+    for _ in range(warmup):
+        pipe.field_pass()
+        pipe.extract_pass()
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    for k in range(steps):
+        ev[k][0].record(stream)
+        pipe.field_pass()
+        ev[k][1].record(stream)
+        pipe.extract_pass()
+        ev[k][2].record(stream)
+    torch.cuda.synchronize()
+    field_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+    extract_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / steps
+    counts = pipe.counts.cpu().tolist()
+    npts = p.npoints()
+    flops = FLOP_PER_POINT[name] * npts
+    ext_bytes = 4 * npts + 12 * counts[0] + 12 * counts[1]
+    ms = field_ms + extract_ms
+    out = {"workload": name, "grid": [p.nx, p.ny, p.nz], "steps": steps,
+           "ms_per_step": round(ms, 4),
+           "gvoxels_per_s": round(npts / (ms * 1e-3) / 1e9, 3),
+           "field_ms": round(field_ms, 4),
+           "field_tflops": round(flops / (field_ms * 1e-3) / 1e12, 2),
+           "field_frac_valu": round(flops / (field_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+           "extract_ms": round(extract_ms, 4),
+           "extract_GBs": round(ext_bytes / (extract_ms * 1e-3) / 1e9, 1),
+           "extract_frac_hbm": round(ext_bytes / (extract_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+           "mtri_per_s": round(counts[1] / (extract_ms * 1e-3) / 1e6, 1),
+           "n_vertices": counts[0], "n_triangles": counts[1]}
+    del pipe
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(p, slab, threads, budget_s=12.0):
@@ -362,6 +404,10 @@ def main():
         rec["fp64_error"] = fp64_errors(p, pipe.field, p.nz // 2 - 16, 33)
     del pipe
     torch.cuda.empty_cache()
+    if not args.no_configs:
+        print("bench: other single-grid workloads", file=sys.stderr, flush=True)
+        rec["other_configs"] = [bench_config(CF.plain_gyroid(128), "plain_gyroid", 20, 3),
+                                bench_config(CF.bent_diamond(1024), "bent_diamond", 5, 1)]
     if not args.no_batch:
         print("bench: batched form", file=sys.stderr, flush=True)
         rec["batched"] = bench_batched(args.batch_runs)

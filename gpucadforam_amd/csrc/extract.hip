@@ -706,6 +706,9 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
         const uint32_t *signs = (const uint32_t *)(a.signs + s.sign_off);
         const uint64_t k = (uint64_t)(gb - s.block_off) * kBlock + part * kTriWords + (tid >> 1);
         const uint32_t *base = a.base + (uint64_t)s.block_off * kBlock * 2;  // by grid half word
+        // first triangle of the wave (loads issued before the sign loads)
+        uint64_t tnext = a.bsum[2 * (uint64_t)gb + 1];
+        for (uint32_t i = 0; i < part * 4 + wave; ++i) tnext += a.twave[kGroups * (uint64_t)gb + i];
         // ---- per half word (owner lane): the sign half of row (y, z) and the
         // crossing halves of the four rows (y..y+1, z..z+1) touching its
         // cells; a cell's cube index and edge vertex numbers follow from these
@@ -725,6 +728,8 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
                 const uint32_t W2 = 2u * (uint32_t)g.W, RW2 = (uint32_t)g.ny * W2;
                 const uint32_t i0 = 2u * (uint32_t)k + h;
                 const bool hn = h == 0 || p.w + 1 < g.W;  // the row has a next half
+                // every load of the half at once (one round trip); a half
+                // without active cells discards them
                 wa = signs[i0];
                 const uint32_t na = ldh(signs, i0 + 1, hn);
                 const uint32_t wb = signs[i0 + W2];
@@ -733,6 +738,13 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
                 const uint32_t nc = ldh(signs, i0 + RW2 + 1, hn);
                 const uint32_t wd = signs[i0 + RW2 + W2];
                 const uint32_t nd = ldh(signs, i0 + RW2 + W2 + 1, hn);
+                const uint32_t we = ldh(signs, i0 + 2 * W2, has_y2),
+                               ne = ldh(signs, i0 + 2 * W2 + 1, has_y2 && hn);
+                const uint32_t wf = ldh(signs, i0 + 2 * W2 + RW2, has_y2);
+                const uint32_t wg = ldh(signs, i0 + 2 * RW2, has_z2);
+                const uint32_t wh = ldh(signs, i0 + 2 * RW2 + W2, has_z2);
+                const uint32_t b0 = base[i0], b1 = base[i0 + W2], b2 = base[i0 + RW2],
+                               b3 = base[i0 + RW2 + W2];
                 const uint32_t sa = sh1(wa, na), sb = sh1(wb, nb), sc = sh1(wc, nc),
                                sd = sh1(wd, nd);
                 act = ((wa ^ sa) | (wa ^ wb) | (wa ^ sb) | (wa ^ wc) | (wa ^ sc) | (wa ^ wd) |
@@ -740,11 +752,6 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
                 if (act) {
                     const uint32_t vpt = below32((int64_t)g.nx - xh);
                     const bool vpt_n = xh + 32 < g.nx, ved_n = xh + 32 < g.nx - 1;
-                    const uint32_t we = ldh(signs, i0 + 2 * W2, has_y2),
-                                   ne = ldh(signs, i0 + 2 * W2 + 1, has_y2 && hn);
-                    const uint32_t wf = ldh(signs, i0 + 2 * W2 + RW2, has_y2);
-                    const uint32_t wg = ldh(signs, i0 + 2 * RW2, has_z2);
-                    const uint32_t wh = ldh(signs, i0 + 2 * RW2 + W2, has_z2);
                     const Cross32 c0 = row_cross32(wa, na, wb, nb, wc, true, true, ved, vpt, ved_n, vpt_n);
                     const Cross32 c1 = row_cross32(wb, nb, we, ne, wd, has_y2, true, ved, vpt, ved_n, vpt_n);
                     const Cross32 c2 = row_cross32(wc, nc, wd, nd, wg, true, has_z2, ved, vpt, ved_n, vpt_n);
@@ -755,10 +762,10 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
                     ex3 = c3.ex; ey3 = c3.ey; ez3 = c3.ez;
                     carry = (c0.sx >> 31) | ((c0.sy >> 31) << 1) | ((c1.sx >> 31) << 2) |
                             ((c1.sy >> 31) << 3) | ((c2.sx >> 31) << 4);
-                    vb0 = base[i0];
-                    vb1 = base[i0 + W2];
-                    vb2 = base[i0 + RW2];
-                    vb3 = base[i0 + RW2 + W2];
+                    vb0 = b0;
+                    vb1 = b1;
+                    vb2 = b2;
+                    vb3 = b3;
                 }
             }
         }
@@ -767,8 +774,6 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
         const uint32_t atot = readlane32(aincl, 63);
         const uint32_t aex = aincl - nact;
         uint16_t *list = s_list[wave];
-        uint64_t tnext = a.bsum[2 * (uint64_t)gb + 1];
-        for (uint32_t i = 0; i < part * 4 + wave; ++i) tnext += a.twave[kGroups * (uint64_t)gb + i];
         uint32_t *vid = s_vid[wave];
         uint64_t *pk = s_pk[wave];
         uint16_t *own = s_own[wave];
