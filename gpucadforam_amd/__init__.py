@@ -53,6 +53,32 @@ class Float4(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
 
 
+class Float2(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float)]
+
+
+class SvlParams(C.Structure):
+    """gc_svl_params: one spatially variant lattice synthesis (include/gpucad.h)."""
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float),
+        ("lattice_type", C.c_char), ("pad", C.c_char * 3),
+        ("uniform_type", C.c_int32),
+        ("const_period", C.c_float), ("x_period", C.c_float), ("y_period", C.c_float),
+        ("z_period", C.c_float),
+        ("lcon", C.c_float), ("lcon_1", C.c_float),
+        ("sinewave_zaxis", C.c_int32),
+        ("range", C.c_int32), ("harmonics", C.c_int32),
+        ("cg_max_iter", C.c_int32), ("cg_end_res", C.c_float),
+        ("fx", C.c_int32), ("fy", C.c_int32), ("fz", C.c_int32),
+        ("fine_dx", C.c_float), ("fine_dy", C.c_float), ("fine_dz", C.c_float),
+    ]
+
+
+class SvlResult(C.Structure):
+    _fields_ = [("cg_iters", C.c_int32 * 3), ("cg_res", C.c_float * 3)]
+
+
 class LatticeParams(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
@@ -252,6 +278,24 @@ SIGNATURES = {
     "gpucad_compute_isosurface": ([_vp, _vp, _f, _u32, _vp, _vp, _vp, _vp, Uint3, Uint3, Uint3,
                                    Float3, Float3, C.POINTER(_u32), C.POINTER(_u32), _vp, _u32,
                                    _vp, _vp, _vp, _f, _f, _u32, C.POINTER(_u64)], C.c_int),
+    "gpucad_svl_period_data": ([_vp, _i32, _i32, _i32, _f, _f, _f, _f, _f, _f, C.c_char],
+                               C.c_int),
+    "gpucad_svl_angle_data": ([_vp, _i32, _i32, _i32, _f, _f, _f, _f, _f, _f, C.c_char], C.c_int),
+    "gpucad_normalise_three": ([_vp, _vp, _u64, _f, _f], C.c_int),
+    "gpucad_normalise_buffer": ([_vp, _vp, _u64], C.c_int),
+    "gpucad_svl_finding_phi": ([_vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _f, _f, _f,
+                                C.c_char, _i32, _f, _f, _f, _f, _f, _f, _i32], C.c_int),
+    "gpucad_svl_matvec": ([_vp, _vp, _i32, _i32, _i32], C.c_int),
+    "gpucad_svl_cg": ([_vp, _i32, _i32, _i32, _i32, _f, C.POINTER(_i32), C.POINTER(_f)],
+                      C.c_int),
+    "gpucad_svl_refine": ([_vp, _i32, _i32, _i32, _vp, _i32, _i32, _i32, _f, _f, _f], C.c_int),
+    "gpucad_svl_grating": ([_vp, _i32, _i32, _i32, _vp, _i32, _i32, _i32, _f, _f, _f], C.c_int),
+    "gpucad_svl_accumulate": ([_vp, _vp, _u64, _i32, _vp], C.c_int),
+    "gpucad_fft_fill": ([_vp, _vp, _i32, _i32, _i32], C.c_int),
+    "gpucad_fft_scale": ([_vp, _f, _u64], C.c_int),
+    "gpucad_unit_spectrum": ([_vp, _i32, _i32, _i32, _i32, _vp], C.c_int),
+    "gpucad_svl_field": ([C.POINTER(SvlParams), _vp, _vp, _vp, C.POINTER(SvlResult), _vp],
+                         C.c_int),
     "gpucad_file_write_obj": ([_vp, _u32, C.c_char_p], C.c_int),
     "gpucad_write_obj_indexed": ([_vp, _u64, _vp, _u64, C.c_char_p, _i32], C.c_int),
 }
@@ -539,3 +583,48 @@ def plain_run_trace(requests, arrivals, iso=0.0):
     wall = C.c_double()
     check(lib.gpucad_plain_run_trace(reqs, arr, n, iso, lat, C.byref(wall)))
     return list(lat), wall.value
+
+
+# ---------------------------------------------------------------------------
+# spatially variant lattice synthesis
+# ---------------------------------------------------------------------------
+def svl_params(n, lattice_type="n", uniform_type=0, const_period=30.0, x_period=30.0,
+               y_period=30.0, z_period=30.0, lcon=0.16, lcon_1=0.6, sinewave_zaxis=False,
+               range_=2, harmonics=None, cg_max_iter=500, cg_end_res=0.01, dx=1.0):
+    """Parameters of the reference's synthesis loop on an n^3 coarse grid
+    (defaults: src/ImguiApp.cpp:91-123, src/main.cu:582-626, 2832-2848):
+    fine grid 2n^3 at half spacing, the first half of the coefficients."""
+    p = SvlParams()
+    p.nx = p.ny = p.nz = n
+    p.dx = p.dy = p.dz = dx
+    p.lattice_type = lattice_type.encode()
+    p.uniform_type = uniform_type
+    p.const_period, p.x_period, p.y_period, p.z_period = const_period, x_period, y_period, z_period
+    p.lcon, p.lcon_1 = lcon, lcon_1
+    p.sinewave_zaxis = 1 if sinewave_zaxis else 0
+    p.range = range_
+    side = 2 * range_ + 1
+    p.harmonics = side ** 3 // 2 if harmonics is None else harmonics
+    p.cg_max_iter, p.cg_end_res = cg_max_iter, cg_end_res
+    p.fx = p.fy = p.fz = 2 * n
+    p.fine_dx = p.fine_dy = p.fine_dz = 0.5 * dx
+    return p
+
+
+def unit_spectrum(cell, n, range_=2):
+    """(2r+1)^3 Fourier coefficients (complex64 tensor) of an n^3 unit cell."""
+    side = 2 * range_ + 1
+    out = torch.empty(side ** 3, dtype=torch.complex64, device=cell.device)
+    check(lib.gpucad_unit_spectrum(ptr(cell), n, n, n, range_, ptr(out)))
+    return out
+
+
+def svl_field(params, coeffs, period=None, stream=None):
+    """Synthesised lattice field on the fine grid (float32 tensor) and the
+    CG statistics of the three unit solves."""
+    out = torch.empty(params.fx * params.fy * params.fz, dtype=torch.float32,
+                      device=coeffs.device)
+    res = SvlResult()
+    check(lib.gpucad_svl_field(C.byref(params), ptr(period), ptr(coeffs), ptr(out), C.byref(res),
+                               stream_handle(stream)))
+    return out, res

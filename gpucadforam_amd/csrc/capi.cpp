@@ -99,7 +99,9 @@ int gpucad_version(void) { return 10000; }
 int gpucad_abi_sizes(uint32_t *sizes, int32_t n) {
     const uint32_t s[] = {(uint32_t)sizeof(gc_lattice_params), (uint32_t)sizeof(gc_extract_args),
                           (uint32_t)sizeof(gc_mesh_counts),    (uint32_t)sizeof(gc_uint3),
-                          (uint32_t)sizeof(gc_float3),         (uint32_t)sizeof(gc_float4)};
+                          (uint32_t)sizeof(gc_float3),         (uint32_t)sizeof(gc_float4),
+                          (uint32_t)sizeof(gc_float2),         (uint32_t)sizeof(gc_svl_params),
+                          (uint32_t)sizeof(gc_svl_result)};
     int m = 0;
     for (; m < n && m < (int)(sizeof(s) / sizeof(s[0])); ++m) sizes[m] = s[m];
     return m;

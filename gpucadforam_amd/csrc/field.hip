@@ -398,6 +398,18 @@ int point_launch(int nx, int ny, int nz, dim3 *grid) {
 
 using namespace gc;
 
+namespace gc {
+// min and max of in[0, n) seeded with 0 (the reference's reductions seed their
+// second pass with 0) into part[0]; part holds 1024 float2
+int enqueue_minmax0(const float *in, uint64_t n, float2 *part, hipStream_t st) {
+    hipLaunchKernelGGL(minmax_partial_kernel, dim3(1024), dim3(256), 0, st, in, n, part);
+    GC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(256), 0, st, part, 1024);
+    GC_CHECK_LAUNCH();
+    return GPUCAD_OK;
+}
+}  // namespace gc
+
 extern "C" {
 
 uint64_t gpucad_sign_words(int32_t nx, int32_t ny, int32_t nz) {

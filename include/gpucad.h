@@ -50,6 +50,7 @@ enum {
 typedef struct gc_uint3 { uint32_t x, y, z; } gc_uint3;
 typedef struct gc_float3 { float x, y, z; } gc_float3;
 typedef struct gc_float4 { float x, y, z, w; } gc_float4;
+typedef struct gc_float2 { float x, y; } gc_float2;
 
 /* ------------------------------------------------------------------------ */
 /* Library                                                                   */
@@ -57,7 +58,8 @@ typedef struct gc_float4 { float x, y, z, w; } gc_float4;
 int gpucad_version(void);                  /* 10000*major + 100*minor + patch */
 /* sizeof of the ABI structs, for binding checks: sizes[0..n) receives
  * gc_lattice_params, gc_extract_args, gc_mesh_counts, gc_uint3, gc_float3,
- * gc_float4 (in that order); returns the number written. */
+ * gc_float4, gc_float2, gc_svl_params, gc_svl_result (in that order);
+ * returns the number written. */
 int gpucad_abi_sizes(uint32_t *sizes, int32_t n);
 const char *gpucad_status_string(int status);
 const char *gpucad_last_error(void);       /* thread-local; "" when none */
@@ -357,6 +359,110 @@ int gpucad_primitive_field_dynamic(const float *primitive_active, float *isosurf
  * src/MarchingCubes_kernel.h:12-18). */
 int gpucad_primitive_field_fixed(const void *primitive_fixed, float *isosurf, int32_t nx,
                                  int32_t ny, int32_t nz);
+
+/* ------------------------------------------------------------------------ */
+/* Spatially variant lattice (SVL) synthesis: the step before meshing       */
+/* (src/main.cu:3904-4037 spatial_lattice_run, 3578-3711 unit_lattice).     */
+/* Synchronous on the null stream unless a stream is taken, like the        */
+/* reference's methods.  Coarse grids need >= 4 points per axis.            */
+/* ------------------------------------------------------------------------ */
+/* Gratings::period_data (src/lattice_files/Gratings.h:36, Gratings.cu:1376,
+ * kernel 815-860): x = (i - mean_x) dx, ...; axis 'z': sqrt((x+1)^2 + (y+1)^2),
+ * 'y': sqrt((x+1)^2 + (z+1)^2), otherwise sqrt((z+1)^2 + (y+1)^2). */
+int gpucad_svl_period_data(float *d_period, int32_t nx, int32_t ny, int32_t nz, float dx, float dy,
+                           float dz, float mean_x, float mean_y, float mean_z, char axis);
+/* Gratings::angle_data (Gratings.h:33, Gratings.cu:1358, kernel 775-812):
+ * atan2(y+1, x+1), or atan2(z+1, x+1) for axis 'y'. */
+int gpucad_svl_angle_data(float *d_theta, int32_t nx, int32_t ny, int32_t nz, float dx, float dy,
+                          float dz, float mean_x, float mean_y, float mean_z, char axis);
+/* Gratings::GPU_buffer_normalise_three (Gratings.h:48, Gratings.cu:1539-1575):
+ * a = min(0, min f), b = max(0, max f), out = a1 + b1 (f-a)/(b-a) (the
+ * reference's [a1, a1+b1] range).  out may alias in. */
+int gpucad_normalise_three(const float *in, float *out, uint64_t size, float a1, float b1);
+/* Gratings::GPU_buffer_normalise_buffer (Gratings.h:46, Gratings.cu:1500-1536):
+ * out = (f-a)/(b-a), a and b as above.  out may alias in. */
+int gpucad_normalise_buffer(const float *in, float *out, uint64_t size);
+/* Gratings::finding_phi (Gratings.h:39-40, kernel Gratings.cu:100-417): the
+ * right-hand side D^T K of harmonic (i, j, k); lattice_type 'n', 'b', 'r'
+ * or 's'; uniform_type 0 (const_period), 1 (x/y/z_period) or 2 (d_period
+ * field, nx*ny*nz). */
+int gpucad_svl_finding_phi(float *d_phi, const float *d_period, int32_t nx, int32_t ny, int32_t nz,
+                           int32_t i, int32_t j, int32_t k, float dx, float dy, float dz,
+                           char lattice_type, int32_t uniform_type, float const_period,
+                           float x_period, float y_period, float z_period, float lcon,
+                           float lcon_1, int32_t sinewave_zaxis);
+/* Gratings::GPUMatvec_lattice (Gratings.h:62, kernel Gratings.cu:420-597):
+ * q = A d, the phase solve's operator. */
+int gpucad_svl_matvec(const float *d_d, float *d_q, int32_t nx, int32_t ny, int32_t nz);
+/* Gratings::GPUCG_lattice (Gratings.h:42, Gratings.cu:875-974): conjugate
+ * gradients from x = 0 on A x = d_phi, the solution written over d_phi;
+ * stops when r.r <= end_res^2 or at `iter` (the reference's iCounter),
+ * final_iter/final_res as the reference reports them.  Scalars stay on the
+ * device (no host round trip per iteration). */
+int gpucad_svl_cg(float *d_phi, int32_t nx, int32_t ny, int32_t nz, int32_t iter, float end_res,
+                  int32_t *final_iter, float *final_res);
+/* Gratings::refine (Gratings.h:67, Gratings.cu:689-722 with the texture of
+ * Interpolations.cu:57-107): trilinear sample of the coarse field at
+ * texel coordinate (f*hx, f*hy, f*hz) per fine point f, clamped (the
+ * reference passes hx = dx2 = 0.5). */
+int gpucad_svl_refine(const float *d_coarse, int32_t nx, int32_t ny, int32_t nz, float *d_fine,
+                      int32_t fx, int32_t fy, int32_t fz, float hx, float hy, float hz);
+/* Gratings::grating (Gratings.h:65, Gratings.cu:653-687): (cos, sin) of the
+ * same sample of phi. */
+int gpucad_svl_grating(const float *d_phi, int32_t nx, int32_t ny, int32_t nz, gc_float2 *d_grating,
+                       int32_t fx, int32_t fy, int32_t fz, float hx, float hy, float hz);
+/* Gratings::svl (Gratings.h:69, Gratings.cu:724-752): svl += Re(grating *
+ * coeffs[index]). */
+int gpucad_svl_accumulate(float *d_svl, const gc_float2 *d_grating, uint64_t n, int32_t index,
+                          const gc_float2 *d_coeffs);
+/* Fft_lattice::fft_fill (Fft_lattice.h:21, Fft_lattice.cu:150-225): the
+ * half spectrum ((nx/2+1) x ny x nz) into the full one with its conjugate
+ * mirror (cubic grids, as the reference's mirror indices assume). */
+int gpucad_fft_fill(const gc_float2 *d_half, gc_float2 *d_full, int32_t nx, int32_t ny,
+                    int32_t nz);
+/* Fft_lattice::fft_scalar (Fft_lattice.h:20, Fft_lattice.cu:124-147): data /= scalar. */
+int gpucad_fft_scale(gc_float2 *d_data, float scalar, uint64_t n);
+
+/* The (2 range + 1)^3 Fourier coefficients of a unit cell that
+ * unit_lattice keeps (src/main.cu:3578-3711: forward real transform scaled
+ * by 1/(nx ny nz), coefficient (i, j, k) at index (i+r) + (2r+1)((j+r) +
+ * (2r+1)(k+r)), loop order k, j, i), by direct DFT with FP64 accumulation.
+ * No reference counterpart as one call (the reference runs a full cuFFT and
+ * picks these on the host); range <= 4. */
+int gpucad_unit_spectrum(const float *d_cell, int32_t nx, int32_t ny, int32_t nz, int32_t range,
+                         gc_float2 *d_coeffs);
+
+/* The whole synthesis loop of spatial_lattice_run in one call (native form).
+ * The right-hand side of finding_phi is linear in (i, j, k), so the phase of
+ * harmonic (i, j, k) is i Phi_x + j Phi_y + k Phi_z with Phi_* the solutions
+ * for the unit wave numbers: three solves (together, to cg_end_res/(3 range)
+ * so every harmonic meets cg_end_res) instead of one per harmonic, then one
+ * pass that upsamples the three phases and sums the first `harmonics`
+ * coefficients (reference order) per fine point. */
+typedef struct gc_svl_params {
+    int32_t nx, ny, nz;          /* coarse grid (NumX, NumY, NumZ), >= 4 each */
+    float dx, dy, dz;            /* coarse spacing (the reference: 1) */
+    char lattice_type;           /* 'n' 'b' 'r' 's' (latticetype_one) */
+    char pad[3];
+    int32_t uniform_type;        /* 0 const, 1 per axis, 2 period field */
+    float const_period, x_period, y_period, z_period;
+    float lcon, lcon_1;          /* the 's' type's sine amplitude and frequency */
+    int32_t sinewave_zaxis;
+    int32_t range;               /* coefficients (2 range + 1)^3, range <= 4 */
+    int32_t harmonics;           /* coefficients summed (the reference: half) */
+    int32_t cg_max_iter;         /* the reference: 500 */
+    float cg_end_res;            /* the reference: 0.01 (on sqrt(r.r)) */
+    int32_t fx, fy, fz;          /* fine grid (2 NumX, ...) */
+    float fine_dx, fine_dy, fine_dz;  /* fine spacing in coarse texels (0.5) */
+} gc_svl_params;
+
+typedef struct gc_svl_result {
+    int32_t cg_iters[3];         /* iterations of the x, y, z unit solves */
+    float cg_res[3];             /* their final sqrt(r.r) */
+} gc_svl_result;
+
+int gpucad_svl_field(const gc_svl_params *p, const float *d_period, const gc_float2 *d_coeffs,
+                     float *d_svl, gc_svl_result *result, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Drop-in extraction entry points (non-indexed float4 output)              */

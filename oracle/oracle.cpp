@@ -1240,4 +1240,347 @@ int orc_file_write_obj(const float *pos4, uint32_t total, const char *filename) 
 
 int orc_max_threads(void) { return omp_get_max_threads(); }
 
+// ===========================================================================
+// Spatially variant lattice synthesis (src/lattice_files/Gratings.cu,
+// src/main.cu:3578-3711, 3904-4037).  The binary32 sequences match the
+// kernels' (same radians-through-revolutions sin/cos, same atan2, the
+// reference's double subexpressions in double); CG sums its dot products in
+// double in index order.
+// ===========================================================================
+static const float kInv2PiF = (float)0.15915494309189533577;
+static const double kTwoPiD = 6.283185307179586476925286766559;
+static inline float cos_rad(float t) { return cos_rev(t * kInv2PiF); }
+static inline float sin_rad(float t) { return sin_rev(t * kInv2PiF); }
+
+// ---- set_period_kernel / set_theta_kernel (Gratings.cu:775-860)
+void orc_svl_period_angle(float *out, int nx, int ny, int nz, float dx, float dy, float dz,
+                          float mx, float my, float mz, int axis, int angle) {
+#pragma omp parallel for collapse(2)
+    for (int zz = 0; zz < nz; ++zz)
+        for (int yy = 0; yy < ny; ++yy)
+            for (int xx = 0; xx < nx; ++xx) {
+                const float x = ((float)xx - mx) * dx, y = ((float)yy - my) * dy,
+                            z = ((float)zz - mz) * dz;
+                float v;
+                if (angle) {
+                    v = axis == 'y' ? atan2_poly(z + 1.0f, x + 1.0f) : atan2_poly(y + 1.0f, x + 1.0f);
+                } else {
+                    float u, w;
+                    if (axis == 'z') { u = x + 1.0f; w = y + 1.0f; }
+                    else if (axis == 'y') { u = x + 1.0f; w = z + 1.0f; }
+                    else { u = z + 1.0f; w = y + 1.0f; }
+                    v = sqrtf(u * u + w * w);
+                }
+                out[pidx(xx, yy, zz, nx, ny)] = v;
+            }
+}
+
+// ---- GPU_buffer_normalise_three / _buffer (Gratings.cu:1500-1575, 1439-1495):
+// min/max seeded with 0, then a1 + b1 k or k
+void orc_svl_rescale(const float *in, float *out, uint64_t n, float a1, float b1, int affine) {
+    float a = 0.f, b = 0.f;
+    for (uint64_t i = 0; i < n; ++i) {
+        a = std::min(a, in[i]);
+        b = std::max(b, in[i]);
+    }
+    for (uint64_t i = 0; i < n; ++i) {
+        const float k = (in[i] - a) / (b - a);
+        out[i] = affine ? a1 + b1 * k : k;
+    }
+}
+
+// ---- finding_phi_kernel (Gratings.cu:100-417)
+static void axis_weights(int c, int n, float *w1, float *w2, int *c1, int *c2) {
+    if (c == 0) { *w1 = -1.0f; *w2 = -0.5f; *c1 = c; *c2 = c + 1; }
+    else if (c == 1) { *w1 = 1.0f; *w2 = -0.5f; *c1 = c - 1; *c2 = c + 1; }
+    else if (c == n - 2) { *w1 = 0.5f; *w2 = -1.0f; *c1 = c - 1; *c2 = c + 1; }
+    else if (c == n - 1) { *w1 = 0.5f; *w2 = 1.0f; *c1 = c - 1; *c2 = c; }
+    else { *w1 = 0.5f; *w2 = -0.5f; *c1 = c - 1; *c2 = c + 1; }
+}
+static inline float wave(float per, float w) { return (float)((kTwoPiD / (double)per) * (double)w); }
+static inline float sine_angle(float lcon, float lcon_1, float u) {
+    const float arg = (float)((6.28 * (double)lcon_1) * (double)u);
+    return lcon * sin_rad(arg);
+}
+
+void orc_svl_finding_phi(float *phi, const float *period, int nx, int ny, int nz, int i, int j,
+                         int k, float dx, float dy, float dz, int type, int uniform_type,
+                         float const_period, float x_period, float y_period, float z_period,
+                         float lcon, float lcon_1, int sinewave_zaxis) {
+#pragma omp parallel for collapse(2)
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                float a1, a2, b1, b2, c1, c2;
+                int x1, x2, y1, y2, z1, z2;
+                axis_weights(x, nx, &a1, &a2, &x1, &x2);
+                axis_weights(y, ny, &b1, &b2, &y1, &y2);
+                axis_weights(z, nz, &c1, &c2, &z1, &z2);
+                float th1 = 0, th2 = 0, th3 = 0, th4 = 0, th5 = 0, th6 = 0, th7 = 0, th8 = 0;
+                if (type == 'r' || type == 'b') {
+                    const float mx = type == 'r' ? (float)(nx + 1) / 2.0f : 0.f;
+                    const float my = type == 'r' ? (float)(ny + 1) / 2.0f : 0.f;
+                    const float yy = ((float)(y + 1) - my) * dx;
+                    th1 = atan2_poly(yy, ((float)(x1 + 1) - mx) * dx);
+                    th2 = atan2_poly(yy, ((float)(x2 + 1) - mx) * dx);
+                    const float xx = ((float)(x + 1) - mx) * dx;
+                    th3 = atan2_poly(((float)(y1 + 1) - my) * dx, xx);
+                    th4 = atan2_poly(((float)(y2 + 1) - my) * dx, xx);
+                } else if (type == 's') {
+                    th1 = sine_angle(lcon, lcon_1, (float)(x1 + 1) * dx);
+                    th5 = sine_angle(lcon, lcon_1, (float)(z1 + 1) * dz);
+                    th2 = sine_angle(lcon, lcon_1, (float)(x2 + 1) * dx);
+                    th6 = sine_angle(lcon, lcon_1, (float)(z2 + 1) * dz);
+                    th3 = sine_angle(lcon, lcon_1, (float)(x + 1) * dx);
+                    th7 = sine_angle(lcon, lcon_1, (float)(z + 1) * dz);
+                    th4 = th3;
+                    th8 = th7;
+                }
+                float p1, p2, p3, p4, p5, p6;
+                if (uniform_type == 0) {
+                    p1 = p2 = p3 = p4 = p5 = p6 = const_period;
+                } else if (uniform_type == 1) {
+                    p1 = p2 = x_period;
+                    p3 = p4 = y_period;
+                    p5 = p6 = z_period;
+                } else {
+                    p1 = period[pidx(x1, y, z, nx, ny)];
+                    p2 = period[pidx(x2, y, z, nx, ny)];
+                    p3 = period[pidx(x, y1, z, nx, ny)];
+                    p4 = period[pidx(x, y2, z, nx, ny)];
+                    p5 = period[pidx(x, y, z1, nx, ny)];
+                    p6 = period[pidx(x, y, z2, nx, ny)];
+                }
+                const float fi = (float)i, fj = (float)j, fk = (float)k;
+                float kx1 = wave(p1, fi * cos_rad(th1) - fj * sin_rad(th1));
+                float kx2 = wave(p2, fi * cos_rad(th2) - fj * sin_rad(th2));
+                float ky1 = wave(p3, fi * sin_rad(th3) + fj * cos_rad(th3));
+                float ky2 = wave(p4, fi * sin_rad(th4) + fj * cos_rad(th4));
+                float kz1 = wave(p5, fk), kz2 = wave(p6, fk);
+                float ph = (fmaf(a1, kx1, a2 * kx2) + fmaf(b1, ky1, b2 * ky2)) +
+                           fmaf(c1, kz1, c2 * kz2);
+                if (type == 's' && sinewave_zaxis) {
+                    kz1 = wave(p5, fj * sin_rad(th5) + fk * cos_rad(th5));
+                    kz2 = wave(p6, fj * sin_rad(th6) + fk * cos_rad(th6));
+                    ky1 = wave(p3, fj * cos_rad(th7) - fk * sin_rad(th7));
+                    ky2 = wave(p4, fj * cos_rad(th8) - fk * sin_rad(th8));
+                    kx1 = wave(p1, fi);
+                    kx2 = wave(p2, fi);
+                    ph += (fmaf(a1, kx1, a2 * kx2) + fmaf(b1, ky1, b2 * ky2)) +
+                          fmaf(c1, kz1, c2 * kz2);
+                }
+                phi[pidx(x, y, z, nx, ny)] = ph;
+            }
+}
+
+// ---- GPUMatvec_lattice_kernel (Gratings.cu:420-597)
+static inline void axis_terms(const float *d, int c, int n, int64_t t, int64_t s, float *w,
+                              float *t2, float *t3) {
+    if (c == 0) { *w = 1.25f; *t2 = -d[t + s]; *t3 = -0.25f * d[t + 2 * s]; }
+    else if (c == 1) { *w = 1.25f; *t2 = -d[t - s]; *t3 = -0.25f * d[t + 2 * s]; }
+    else if (c == n - 2) { *w = 1.25f; *t2 = -0.25f * d[t - 2 * s]; *t3 = -d[t + s]; }
+    else if (c == n - 1) { *w = 1.25f; *t2 = -0.25f * d[t - 2 * s]; *t3 = -d[t - s]; }
+    else { *w = 0.5f; *t2 = -0.25f * d[t - 2 * s]; *t3 = -0.25f * d[t + 2 * s]; }
+}
+
+void orc_svl_matvec(const float *d, float *q, int nx, int ny, int nz) {
+#pragma omp parallel for collapse(2)
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const int64_t t = (int64_t)pidx(x, y, z, nx, ny);
+                float wx, x2, x3, wy, y2, y3, wz, z2, z3;
+                axis_terms(d, x, nx, t, 1, &wx, &x2, &x3);
+                axis_terms(d, y, ny, t, nx, &wy, &y2, &y3);
+                axis_terms(d, z, nz, t, (int64_t)nx * ny, &wz, &z2, &z3);
+                const float w = (wx + wy) + wz;
+                float a = fmaf(w, d[t], x2);
+                a = a + x3;
+                a = a + y2;
+                a = a + y3;
+                a = a + z2;
+                a = a + z3;
+                q[t] = a;
+            }
+}
+
+static double dot_d(const float *a, const float *b, uint64_t n) {
+    double s = 0.0;
+    for (uint64_t i = 0; i < n; ++i) s += (double)a[i] * (double)b[i];
+    return s;
+}
+
+// ---- GPUCG_lattice (Gratings.cu:875-974): x = 0, r = d = b; while
+// (iCounter < iter && r.r > end_res^2) { q = A d; alpha = r.r / d.q;
+// x += alpha d; r -= alpha q; beta = r.r_new / r.r_old; d = r + beta d }
+void orc_svl_cg(float *b, int nx, int ny, int nz, int iter, float end_res, int *final_iter,
+                float *final_res) {
+    const uint64_t n = (uint64_t)nx * ny * nz;
+    std::vector<float> x(n, 0.f), r(b, b + n), d(b, b + n), q(n);
+    float delta = (float)dot_d(r.data(), d.data(), n);
+    const float term = end_res * end_res;
+    int it = 1;
+    while (it < iter && delta > term) {
+        orc_svl_matvec(d.data(), q.data(), nx, ny, nz);
+        const float alpha = delta / (float)dot_d(d.data(), q.data(), n);
+        for (uint64_t i = 0; i < n; ++i) {
+            x[i] = fmaf(alpha, d[i], x[i]);
+            r[i] = fmaf(-alpha, q[i], r[i]);
+        }
+        const float old = delta;
+        delta = (float)dot_d(r.data(), r.data(), n);
+        const float beta = delta / old;
+        for (uint64_t i = 0; i < n; ++i) d[i] = fmaf(beta, d[i], r[i]);
+        ++it;
+    }
+    std::copy(x.begin(), x.end(), b);
+    *final_iter = it;
+    *final_res = sqrtf(delta);
+}
+
+// ---- refine / grating texture sampling (Gratings.cu:653-722,
+// Interpolations.cu:57-107): texel coordinate f*h, weights quantised to
+// 1/256, clamped, lerps x then y then z
+struct OTap {
+    int64_t i0, i1;
+    float w;
+};
+static OTap otap(int f, float h, int n) {
+    const float u = (float)f * h;
+    const float fl = floorf(u);
+    float w = u - fl;
+    w = rintf(w * 256.0f) * (1.0f / 256.0f);
+    int64_t i0 = (int64_t)fl;
+    if (w >= 1.0f) {
+        i0 += 1;
+        w = 0.0f;
+    }
+    OTap t;
+    t.i0 = std::min<int64_t>(std::max<int64_t>(i0, 0), n - 1);
+    t.i1 = std::min<int64_t>(std::max<int64_t>(i0 + 1, 0), n - 1);
+    t.w = w;
+    return t;
+}
+static inline float olerp(float a, float b, float w) { return fmaf(w, b, (1.0f - w) * a); }
+static float osample(const float *c, const OTap &tx, const OTap &ty, const OTap &tz, int64_t row,
+                     int64_t plane) {
+    const int64_t z0 = tz.i0 * plane, z1 = tz.i1 * plane, y0 = ty.i0 * row, y1 = ty.i1 * row;
+    const float c00 = olerp(c[z0 + y0 + tx.i0], c[z0 + y0 + tx.i1], tx.w);
+    const float c10 = olerp(c[z0 + y1 + tx.i0], c[z0 + y1 + tx.i1], tx.w);
+    const float c01 = olerp(c[z1 + y0 + tx.i0], c[z1 + y0 + tx.i1], tx.w);
+    const float c11 = olerp(c[z1 + y1 + tx.i0], c[z1 + y1 + tx.i1], tx.w);
+    return olerp(olerp(c00, c10, ty.w), olerp(c01, c11, ty.w), tz.w);
+}
+
+// mode 0: out[f] = sample; mode 1: out[2f], out[2f+1] = cos, sin of the sample
+void orc_svl_upsample(const float *c, int nx, int ny, int nz, float *out, int fx, int fy, int fz,
+                      float hx, float hy, float hz, int mode) {
+#pragma omp parallel for collapse(2)
+    for (int z = 0; z < fz; ++z)
+        for (int y = 0; y < fy; ++y)
+            for (int x = 0; x < fx; ++x) {
+                const float v = osample(c, otap(x, hx, nx), otap(y, hy, ny), otap(z, hz, nz), nx,
+                                        (int64_t)nx * ny);
+                const uint64_t t = pidx(x, y, z, fx, fy);
+                if (mode == 0) out[t] = v;
+                else {
+                    out[2 * t] = cos_rad(v);
+                    out[2 * t + 1] = sin_rad(v);
+                }
+            }
+}
+
+// ---- svl_kernel (Gratings.cu:724-752)
+void orc_svl_accumulate(float *svl, const float *grating, uint64_t n, int index,
+                        const float *coeffs) {
+    const float cr = coeffs[2 * index], ci = coeffs[2 * index + 1];
+    for (uint64_t t = 0; t < n; ++t)
+        svl[t] = svl[t] + fmaf(grating[2 * t], cr, -(grating[2 * t + 1] * ci));
+}
+
+// ---- the synthesis loop of spatial_lattice_run as one field: the three unit
+// phases (each solved alone, to end_res / (3 range)), then per fine point the
+// sum over the first `harmonics` coefficients of Re(c e^{i phi}) with phi =
+// i Phi_x + j Phi_y + k Phi_z evaluated directly
+void orc_svl_field(int nx, int ny, int nz, float dx, float dy, float dz, int type,
+                   int uniform_type, float const_period, float x_period, float y_period,
+                   float z_period, float lcon, float lcon_1, int sinewave_zaxis,
+                   const float *period, const float *coeffs, int range, int harmonics,
+                   int cg_max_iter, float cg_end_res, int fx, int fy, int fz, float hx, float hy,
+                   float hz, float *svl, int *iters, float *res) {
+    const uint64_t n = (uint64_t)nx * ny * nz;
+    std::vector<float> ph(3 * n);
+    const int unit[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    const float tol = cg_end_res / (float)(3 * std::max(1, range));
+    for (int a = 0; a < 3; ++a) {
+        orc_svl_finding_phi(ph.data() + a * n, period, nx, ny, nz, unit[a][0], unit[a][1],
+                            unit[a][2], dx, dy, dz, type, uniform_type, const_period, x_period,
+                            y_period, z_period, lcon, lcon_1, sinewave_zaxis);
+        orc_svl_cg(ph.data() + a * n, nx, ny, nz, cg_max_iter, tol, iters + a, res + a);
+    }
+    const int side = 2 * range + 1;
+#pragma omp parallel for collapse(2)
+    for (int z = 0; z < fz; ++z)
+        for (int y = 0; y < fy; ++y)
+            for (int x = 0; x < fx; ++x) {
+                const OTap tx = otap(x, hx, nx), ty = otap(y, hy, ny), tz = otap(z, hz, nz);
+                const int64_t row = nx, plane = (int64_t)nx * ny;
+                const float px = osample(ph.data(), tx, ty, tz, row, plane);
+                const float py = osample(ph.data() + n, tx, ty, tz, row, plane);
+                const float pz = osample(ph.data() + 2 * n, tx, ty, tz, row, plane);
+                float acc = 0.f;
+                for (int h = 0; h < harmonics; ++h) {
+                    const int i = h % side - range, j = (h / side) % side - range,
+                              k = h / (side * side) - range;
+                    const float phi = ((float)i * px + (float)j * py) + (float)k * pz;
+                    acc = acc + fmaf(cos_rad(phi), coeffs[2 * h], -(sin_rad(phi) * coeffs[2 * h + 1]));
+                }
+                svl[pidx(x, y, z, fx, fy)] = acc;
+            }
+}
+
+// ---- unit_lattice's coefficients (src/main.cu:3578-3711 with Fft_lattice.cu
+// fft_scalar): (2r+1)^3 DFT coefficients / N, loop order k, j, i, in double
+void orc_unit_spectrum(const float *cell, int nx, int ny, int nz, int range, double *out) {
+    const int side = 2 * range + 1;
+#pragma omp parallel for
+    for (int h = 0; h < side * side * side; ++h) {
+        const int i = h % side - range, j = (h / side) % side - range, k = h / (side * side) - range;
+        double re = 0.0, im = 0.0;
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    const double ang = -kTwoPiD * ((double)i * x / nx + (double)j * y / ny +
+                                                   (double)k * z / nz);
+                    const double f = cell[pidx(x, y, z, nx, ny)];
+                    re += f * cos(ang);
+                    im += f * sin(ang);
+                }
+        const double N = (double)nx * ny * nz;
+        out[2 * h] = re / N;
+        out[2 * h + 1] = im / N;
+    }
+}
+
+// ---- fft_fill_kernel (Fft_lattice.cu:150-225)
+void orc_fft_fill(const float *half, float *full, int nx, int ny, int nz) {
+    const int nx2 = nx / 2 + 1;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx2; ++x) {
+                const uint64_t t = (uint64_t)x + (uint64_t)y * nx2 + (uint64_t)z * nx2 * ny;
+                if (t == 0) {
+                    full[0] = half[0];
+                    full[1] = half[1];
+                    continue;
+                }
+                const int e = x == 0 ? 0 : nx - x, f = y == 0 ? 0 : nx - y, g = z == 0 ? 0 : nx - z;
+                const uint64_t d1 = pidx(x, y, z, nx, ny), d2 = pidx(e, f, g, nx, ny);
+                full[2 * d1] = half[2 * t];
+                full[2 * d1 + 1] = half[2 * t + 1];
+                full[2 * d2] = half[2 * t];
+                full[2 * d2 + 1] = -half[2 * t + 1];
+            }
+}
+
 }  // extern "C"

@@ -64,6 +64,27 @@ def _load():
         "orc_bool_generate": ([C.c_uint, vp, vp, vp, C.c_int, C.c_int, C.c_int, f32p, f32p,
                                C.c_float, C.c_float, C.c_float, u32p, u32p, C.c_uint32,
                                C.c_uint32, f32p, f32p], None),
+        "orc_svl_period_angle": ([f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                  C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int],
+                                 None),
+        "orc_svl_rescale": ([f32p, f32p, C.c_uint64, C.c_float, C.c_float, C.c_int], None),
+        "orc_svl_finding_phi": ([f32p, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float,
+                                 C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int],
+                                None),
+        "orc_svl_matvec": ([f32p, f32p, C.c_int, C.c_int, C.c_int], None),
+        "orc_svl_cg": ([f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                        C.POINTER(C.c_int), C.POINTER(C.c_float)], None),
+        "orc_svl_upsample": ([f32p, C.c_int, C.c_int, C.c_int, f32p, C.c_int, C.c_int, C.c_int,
+                              C.c_float, C.c_float, C.c_float, C.c_int], None),
+        "orc_svl_accumulate": ([f32p, f32p, C.c_uint64, C.c_int, f32p], None),
+        "orc_svl_field": ([C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int,
+                           C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                           C.c_float, C.c_int, vp, f32p, C.c_int, C.c_int, C.c_int, C.c_float,
+                           C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, f32p,
+                           np.ctypeslib.ndpointer(np.int32, flags="C"), f32p], None),
+        "orc_unit_spectrum": ([f32p, C.c_int, C.c_int, C.c_int, C.c_int, f64p], None),
+        "orc_fft_fill": ([f32p, f32p, C.c_int, C.c_int, C.c_int], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -286,3 +307,98 @@ def bool_compute_isosurface(flags, fixed, dyn, lat, nx, ny, nz, iso, iso1, iso2,
     out.update(comp=comp, verts_scan=verts_scan, total=total, nfacets=total // 3,
                pos=pos.reshape(-1, 4), norm=norm.reshape(-1, 4))
     return out
+
+
+# ---------------------------------------------------------------------------
+# spatially variant lattice synthesis (src/lattice_files/Gratings.cu,
+# src/main.cu:3578-3711, 3904-4037)
+# ---------------------------------------------------------------------------
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def svl_period_angle(nx, ny, nz, dx, dy, dz, mean, axis="z", angle=False):
+    out = np.empty(nx * ny * nz, np.float32)
+    lib.orc_svl_period_angle(out, nx, ny, nz, dx, dy, dz, mean[0], mean[1], mean[2], ord(axis),
+                             1 if angle else 0)
+    return out
+
+
+def svl_rescale(a, a1=0.0, b1=0.0, affine=True):
+    a = _f32(a).ravel()
+    out = np.empty_like(a)
+    lib.orc_svl_rescale(a, out, a.size, a1, b1, 1 if affine else 0)
+    return out
+
+
+def svl_finding_phi(nx, ny, nz, i, j, k, lattice_type="n", uniform_type=0, const_period=30.0,
+                    periods=(30.0, 30.0, 30.0), period=None, d=(1.0, 1.0, 1.0), lcon=0.16,
+                    lcon_1=0.6, sinewave_zaxis=False):
+    out = np.empty(nx * ny * nz, np.float32)
+    keep = None if period is None else _f32(period)
+    per = None if keep is None else keep.ctypes.data
+    lib.orc_svl_finding_phi(out, per, nx, ny, nz, i, j, k, d[0], d[1], d[2], ord(lattice_type),
+                            uniform_type, const_period, periods[0], periods[1], periods[2], lcon,
+                            lcon_1, 1 if sinewave_zaxis else 0)
+    return out
+
+
+def svl_matvec(d, nx, ny, nz):
+    d = _f32(d).ravel()
+    q = np.empty_like(d)
+    lib.orc_svl_matvec(d, q, nx, ny, nz)
+    return q
+
+
+def svl_cg(b, nx, ny, nz, iters=500, end_res=0.01):
+    x = _f32(b).ravel().copy()
+    it, res = C.c_int(0), C.c_float(0)
+    lib.orc_svl_cg(x, nx, ny, nz, iters, end_res, C.byref(it), C.byref(res))
+    return x, it.value, res.value
+
+
+def svl_upsample(c, nx, ny, nz, fine, h, mode=0):
+    c = _f32(c).ravel()
+    fx, fy, fz = fine
+    out = np.empty(fx * fy * fz * (2 if mode else 1), np.float32)
+    lib.orc_svl_upsample(c, nx, ny, nz, out, fx, fy, fz, h[0], h[1], h[2], mode)
+    return out
+
+
+def svl_accumulate(svl, grating, index, coeffs):
+    svl = _f32(svl).ravel().copy()
+    g = _f32(grating).ravel()
+    c = _f32(coeffs).ravel()
+    lib.orc_svl_accumulate(svl, g, svl.size, index, c)
+    return svl
+
+
+def svl_field(p, coeffs, period=None):
+    """The synthesis of a gc_svl_params-like object p (attributes as in
+    gpucadforam_amd.SvlParams) with coefficients as (N, 2) float32."""
+    c = _f32(coeffs).ravel()
+    out = np.empty(p.fx * p.fy * p.fz, np.float32)
+    it = np.zeros(3, np.int32)
+    res = np.zeros(3, np.float32)
+    keep = None if period is None else _f32(period)
+    lt = p.lattice_type if isinstance(p.lattice_type, int) else ord(p.lattice_type)
+    lib.orc_svl_field(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, lt, p.uniform_type, p.const_period,
+                      p.x_period, p.y_period, p.z_period, p.lcon, p.lcon_1, p.sinewave_zaxis,
+                      None if keep is None else keep.ctypes.data, c, p.range, p.harmonics,
+                      p.cg_max_iter, p.cg_end_res, p.fx, p.fy, p.fz, p.fine_dx, p.fine_dy,
+                      p.fine_dz, out, it, res)
+    return out, it, res
+
+
+def unit_spectrum(cell, nx, ny, nz, range_=2):
+    side = 2 * range_ + 1
+    out = np.empty(2 * side ** 3, np.float64)
+    lib.orc_unit_spectrum(_f32(cell).ravel(), nx, ny, nz, range_, out)
+    return out[0::2] + 1j * out[1::2]
+
+
+def fft_fill(half, nx, ny, nz):
+    h = _f32(half).ravel()
+    full = np.zeros(2 * nx * ny * nz, np.float32)
+    lib.orc_fft_fill(h, full, nx, ny, nz)
+    return full

@@ -35,12 +35,13 @@ def test_binding_covers_every_declared_symbol(gc):
 
 
 def test_struct_layouts_match(gc):
-    sizes = (C.c_uint32 * 6)()
-    assert gc.lib.gpucad_abi_sizes(sizes, 6) == 6
+    sizes = (C.c_uint32 * 9)()
+    assert gc.lib.gpucad_abi_sizes(sizes, 9) == 9
     expect = [C.sizeof(gc.LatticeParams), C.sizeof(gc.ExtractArgs), C.sizeof(gc.MeshCounts),
-              C.sizeof(gc.Uint3), C.sizeof(gc.Float3), C.sizeof(gc.Float4)]
+              C.sizeof(gc.Uint3), C.sizeof(gc.Float3), C.sizeof(gc.Float4), C.sizeof(gc.Float2),
+              C.sizeof(gc.SvlParams), C.sizeof(gc.SvlResult)]
     assert list(sizes) == expect
-    assert list(sizes)[3:] == [12, 12, 16]
+    assert list(sizes)[3:7] == [12, 12, 16, 8]
 
 
 def test_version_and_status_strings(gc):
