@@ -288,6 +288,9 @@ SIGNATURES = {
     "gpucad_svl_matvec": ([_vp, _vp, _i32, _i32, _i32], C.c_int),
     "gpucad_svl_cg": ([_vp, _i32, _i32, _i32, _i32, _f, C.POINTER(_i32), C.POINTER(_f)],
                       C.c_int),
+    "gpucad_svl_axpby": ([_vp, _f, _vp, _f, _u64], C.c_int),
+    "gpucad_svl_dot": ([_vp, _vp, _vp, _u64], C.c_int),
+    "gpucad_svl_fillfrac": ([_vp, _vp, _i32, _i32, _i32], C.c_int),
     "gpucad_svl_refine": ([_vp, _i32, _i32, _i32, _vp, _i32, _i32, _i32, _f, _f, _f], C.c_int),
     "gpucad_svl_grating": ([_vp, _i32, _i32, _i32, _vp, _i32, _i32, _i32, _f, _f, _f], C.c_int),
     "gpucad_svl_accumulate": ([_vp, _vp, _u64, _i32, _vp], C.c_int),

@@ -658,6 +658,36 @@ __global__ void fft_scale_kernel(float2 *d, float s, uint64_t n) {
     }
 }
 
+// VecSMultAddKernel_lattice (Gratings.cu:73-94): v = a1 v + a2 w (fused)
+__global__ void axpby_kernel(float *v, float a1, const float *w, float a2, uint64_t n) {
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * blockDim.x)
+        v[t] = __builtin_fmaf(a2, w[t], a1 * v[t]);
+}
+
+// fillfrac_kernel (Gratings.cu:755-772): svl *= fillfrac
+__global__ void fillfrac_kernel(float *svl, const float *frac, uint64_t n) {
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * blockDim.x)
+        svl[t] = frac[t] * svl[t];
+}
+
+// GPUScalar_lattice (Gratings.cu:600-650, 22-70): a.b into result[0]
+// (partial sums in double per block, one fixed-order final sum)
+__global__ void dot_partial_kernel(const float *a, const float *b, uint64_t n, double *part) {
+    double acc = 0.0;
+    for (uint64_t t = (uint64_t)blockIdx.x * kCgThreads + threadIdx.x; t < n;
+         t += (uint64_t)gridDim.x * kCgThreads)
+        acc += (double)a[t] * (double)b[t];
+    const double tot = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ void dot_final_kernel(const double *part, int G, float *result) {
+    const double tot = sum_partials(part, G);
+    if (threadIdx.x == 0) result[0] = (float)tot;
+}
+
 dim3 stride_grid(uint64_t n) {
     return dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 8192)));
 }
@@ -822,6 +852,45 @@ int gpucad_svl_cg(float *d_phi, int32_t nx, int32_t ny, int32_t nz, int32_t iter
     if (st) return st;
     if (final_iter) *final_iter = it;
     if (final_res) *final_res = res;
+    return GPUCAD_OK;
+}
+
+int gpucad_svl_axpby(float *d_v, float a1, const float *d_w, float a2, uint64_t n) {
+    clear_error();
+    GC_ARG(d_v && d_w, "NULL buffer");
+    hipLaunchKernelGGL(axpby_kernel, stride_grid(n), dim3(256), 0, 0, d_v, a1, d_w, a2, n);
+    GC_CHECK_LAUNCH();
+    GC_HIP(hipStreamSynchronize(0));
+    return GPUCAD_OK;
+}
+
+int gpucad_svl_dot(float *d_result, const float *d_a, const float *d_b, uint64_t n) {
+    clear_error();
+    GC_ARG(d_result && d_a && d_b, "NULL buffer");
+    const int G = (int)std::max<uint64_t>(1, std::min<uint64_t>((n + kCgThreads - 1) / kCgThreads,
+                                                                kCgBlocks));
+    void *ws = nullptr;
+    int st = scratch(G * sizeof(double), &ws);
+    if (st) return st;
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(G), dim3(kCgThreads), 0, 0, d_a, d_b, n,
+                       (double *)ws);
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kCgThreads), 0, 0, (const double *)ws, G,
+                       d_result);
+    GC_CHECK_LAUNCH();
+    GC_HIP(hipStreamSynchronize(0));
+    return GPUCAD_OK;
+}
+
+int gpucad_svl_fillfrac(float *d_svl, const float *d_fillfrac, int32_t nx, int32_t ny,
+                        int32_t nz) {
+    clear_error();
+    GC_ARG(d_svl && d_fillfrac, "NULL buffer");
+    int st = check_dims(nx, ny, nz, 1);
+    if (st) return st;
+    const uint64_t n = (uint64_t)nx * ny * nz;
+    hipLaunchKernelGGL(fillfrac_kernel, stride_grid(n), dim3(256), 0, 0, d_svl, d_fillfrac, n);
+    GC_CHECK_LAUNCH();
+    GC_HIP(hipStreamSynchronize(0));
     return GPUCAD_OK;
 }
 

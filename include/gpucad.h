@@ -401,6 +401,15 @@ int gpucad_svl_matvec(const float *d_d, float *d_q, int32_t nx, int32_t ny, int3
  * device (no host round trip per iteration). */
 int gpucad_svl_cg(float *d_phi, int32_t nx, int32_t ny, int32_t nz, int32_t iter, float end_res,
                   int32_t *final_iter, float *final_res);
+/* Gratings::VecSMultAdd_lattice (Gratings.h:37, Gratings.cu:73-94, 977-985):
+ * v = a1 v + a2 w over n points. */
+int gpucad_svl_axpby(float *d_v, float a1, const float *d_w, float a2, uint64_t n);
+/* Gratings::GPUScalar_lattice (Gratings.h:44, Gratings.cu:600-650, 1032-1048):
+ * result[0] = a.b (binary32 result of a double sum). */
+int gpucad_svl_dot(float *d_result, const float *d_a, const float *d_b, uint64_t n);
+/* Gratings::fillfrac (Gratings.h:71, Gratings.cu:755-772, 1000-1006): svl *= fillfrac. */
+int gpucad_svl_fillfrac(float *d_svl, const float *d_fillfrac, int32_t nx, int32_t ny,
+                        int32_t nz);
 /* Gratings::refine (Gratings.h:67, Gratings.cu:689-722 with the texture of
  * Interpolations.cu:57-107): trilinear sample of the coarse field at
  * texel coordinate (f*hx, f*hy, f*hz) per fine point f, clamped (the

@@ -7,8 +7,12 @@
 //   MarchingCubeCuda   src/MarchingCubes_kernel.h:34-124 (path methods)
 //   Isosurface         src/Isosurface.h:10-80 (lattice and boolean extraction)
 //   Modelling          src/Modelling.h:10-48 (primitive fields)
-//   Fft_lattice        src/lattice_files/Fft_lattice.h:10-24 (create_lattice)
-//   Gratings           src/lattice_files/Gratings.h:10-80 (normalise, masks)
+//   Fft_lattice        src/lattice_files/Fft_lattice.h:10-24 (create_lattice,
+//                      fft_scalar, fft_fill)
+//   Interpolations     src/Interpolations.h:10-40 (the phase texture of the
+//                      lattice synthesis)
+//   Gratings           src/lattice_files/Gratings.h:10-80 (lattice synthesis,
+//                      normalise, masks)
 //   File_output        src/File_output.h:35-44 (file_write_obj)
 //
 // Semantics follow the reference: every call is synchronous; launch-shape
@@ -375,11 +379,148 @@ public:
             gpucad_create_lattice(d_latticevol, NX, NY, NZ, size, lattice_type_index),
             "create_lattice");
     }
+    // The transforms themselves (fft_func, ifft_func: cuFFT plans owned by
+    // src/main.cu:78-80, 3522-3526) are library calls of the caller;
+    // gpucad_unit_spectrum gives the coefficients the synthesis needs directly.
+    void fft_scalar(float2 *fft_data_compute, float scalar_val, int size) {
+        gpucad_compat::check(gpucad_fft_scale((gc_float2 *)fft_data_compute, scalar_val, size),
+                             "fft_scalar");
+    }
+    void fft_fill(float2 *fft_compute, float2 *fft_compute_fill, int Nx, int Ny, int Nz,
+                  uint mid_index) {
+        (void)mid_index;
+        gpucad_compat::check(gpucad_fft_fill((const gc_float2 *)fft_compute,
+                                             (gc_float2 *)fft_compute_fill, Nx, Ny, Nz),
+                             "fft_fill");
+    }
 };
 
-class Gratings {
+// The reference samples the phase through a 3-D texture (linear filter,
+// unnormalised coordinates, src/Interpolations.cu:57-107).  Here the
+// "texture" is a plain device copy of the phase taken at updateTexture, and
+// Gratings::grating/refine sample it with the same filter arithmetic.
+class Interpolations {
+protected:
+    float *tex_ = nullptr;  // the texture's contents
+    int tex_x_ = 0, tex_y_ = 0, tex_z_ = 0;
+
+public:
+    void setupTexture(int x, int y, int z) {
+        deleteTexture();
+        tex_x_ = x;
+        tex_y_ = y;
+        tex_z_ = z;
+        gpucad_compat::check(
+            hipMalloc((void **)&tex_, sizeof(float) * (size_t)x * y * z) == hipSuccess
+                ? GPUCAD_OK
+                : GPUCAD_E_OOM,
+            "setupTexture");
+    }
+    // writes d_phi into the pitched staging buffer, as the reference does
+    void copytotexture(float *d_phi, hipPitchedPtr data_ptr, int NX, int NY, int NZ) {
+        gpucad_compat::check(
+            hipMemcpy2D(data_ptr.ptr, data_ptr.pitch, d_phi, sizeof(float) * NX,
+                        sizeof(float) * NX, (size_t)NY * NZ, hipMemcpyDeviceToDevice) == hipSuccess
+                ? GPUCAD_OK
+                : GPUCAD_E_HIP,
+            "copytotexture");
+    }
+    // the staging buffer becomes the texture's contents
+    void updateTexture(hipPitchedPtr data_ptr) {
+        gpucad_compat::check(
+            hipMemcpy2D(tex_, sizeof(float) * tex_x_, data_ptr.ptr, data_ptr.pitch,
+                        sizeof(float) * tex_x_, (size_t)tex_y_ * tex_z_,
+                        hipMemcpyDeviceToDevice) == hipSuccess
+                ? GPUCAD_OK
+                : GPUCAD_E_HIP,
+            "updateTexture");
+    }
+    void deleteTexture() {
+        if (tex_) (void)hipFree(tex_);
+        tex_ = nullptr;
+    }
+};
+
+class Gratings : public Interpolations {
 public:
     int NX = 0, NY = 0, NZ = 0;
+
+    void angle_data(float *d_theta, int NX_, int NY_, int NZ_, float dx, float dy, float dz,
+                    float mean_x, float mean_y, float mean_z, char axis) {
+        gpucad_compat::check(gpucad_svl_angle_data(d_theta, NX_, NY_, NZ_, dx, dy, dz, mean_x,
+                                                   mean_y, mean_z, axis),
+                             "angle_data");
+    }
+    void period_data(float *d_period, int NX_, int NY_, int NZ_, float dx, float dy, float dz,
+                     float mean_x, float mean_y, float mean_z, char axis) {
+        gpucad_compat::check(gpucad_svl_period_data(d_period, NX_, NY_, NZ_, dx, dy, dz, mean_x,
+                                                    mean_y, mean_z, axis),
+                             "period_data");
+    }
+    void VecSMultAdd_lattice(float *d_v, float a1, float *d_w, const float a2, const int NX_,
+                             const int NY_, const int NZ_) {
+        gpucad_compat::check(gpucad_svl_axpby(d_v, a1, d_w, a2, (uint64_t)NX_ * NY_ * NZ_),
+                             "VecSMultAdd_lattice");
+    }
+    void finding_phi(float *d_phi, float *d_period, int x_dim, int y_dim, int z_dim, int i,
+                     int j, int k, float dx, float dy, float dz, char latticetype_one,
+                     int unform_type, float const_peirod, float x_period, float y_period,
+                     float z_period, float lcon, float lcon_1, bool sinewave_zaxis) {
+        gpucad_compat::check(
+            gpucad_svl_finding_phi(d_phi, d_period, x_dim, y_dim, z_dim, i, j, k, dx, dy, dz,
+                                   latticetype_one, unform_type, const_peirod, x_period,
+                                   y_period, z_period, lcon, lcon_1, sinewave_zaxis ? 1 : 0),
+            "finding_phi");
+    }
+    // on the object's NX, NY, NZ, as the reference's
+    void GPUCG_lattice(float *d_phi, const int iter, const int OptIter, const float EndRes,
+                       int &FinalIter, float &FinalRes) {
+        (void)OptIter;
+        int32_t it = 0;
+        float res = 0.f;
+        gpucad_compat::check(gpucad_svl_cg(d_phi, NX, NY, NZ, iter, EndRes, &it, &res),
+                             "GPUCG_lattice");
+        FinalIter = it;
+        FinalRes = res;
+    }
+    void GPUScalar_lattice(float *d_result, float *d_vec1, float *d_vec2, int n, int block_num) {
+        (void)block_num;
+        gpucad_compat::check(gpucad_svl_dot(d_result, d_vec1, d_vec2, (uint64_t)n),
+                             "GPUScalar_lattice");
+    }
+    void GPU_buffer_normalise_buffer(float *d_vec1, float *d_vec2, int n) {
+        gpucad_compat::check(gpucad_normalise_buffer(d_vec1, d_vec2, (uint64_t)n),
+                             "GPU_buffer_normalise_buffer");
+    }
+    void GPU_buffer_normalise_three(float *dataone, float *datatwo, size_t size, float a1,
+                                    float b1) {
+        gpucad_compat::check(gpucad_normalise_three(dataone, datatwo, size, a1, b1),
+                             "GPU_buffer_normalise_three");
+    }
+    void GPUMatvec_lattice(float *d_d, float *d_q, int NX_, int NY_, int NZ_) {
+        gpucad_compat::check(gpucad_svl_matvec(d_d, d_q, NX_, NY_, NZ_), "GPUMatvec_lattice");
+    }
+    void grating(float2 *dvol, int NX2, int NY2, int NZ2, float dx2, float dy2, float dz2) {
+        gpucad_compat::check(gpucad_svl_grating(tex_, tex_x_, tex_y_, tex_z_, (gc_float2 *)dvol,
+                                                NX2, NY2, NZ2, dx2, dy2, dz2),
+                             "grating");
+    }
+    void refine(float *dvol, int NX2, int NY2, int NZ2, float dx, float dy, float dz) {
+        gpucad_compat::check(gpucad_svl_refine(tex_, tex_x_, tex_y_, tex_z_, dvol, NX2, NY2, NZ2,
+                                               dx, dy, dz),
+                             "refine");
+    }
+    void svl(float *d_svl, float2 *d_grating, int NX_, int NY_, int NZ_, int indxx,
+             float2 *data_fft) {
+        gpucad_compat::check(gpucad_svl_accumulate(d_svl, (const gc_float2 *)d_grating,
+                                                   (uint64_t)NX_ * NY_ * NZ_, indxx,
+                                                   (const gc_float2 *)data_fft),
+                             "svl");
+    }
+    void fillfrac(float *d_svl, float *d_fillfrac, int NX_, int NY_, int NZ_) {
+        gpucad_compat::check(gpucad_svl_fillfrac(d_svl, d_fillfrac, NX_, NY_, NZ_), "fillfrac");
+    }
+    void GPUCleanUp() {}
 
     void GPU_buffer_normalise_four(float *dataone, float *datatwo, float *datathree, size_t size,
                                    int Nx, int Ny, int Nz, float isoval_1, float isoval_2) {

@@ -145,6 +145,24 @@ def test_accumulate_bit_exact(gc, orc, dev):
     assert np.array_equal(ds.cpu().numpy(), orc.svl_accumulate(svl, g, 13, c))
 
 
+def test_axpby_fillfrac_dot(gc, dev):
+    rng = np.random.default_rng(17)
+    n = 70001
+    v = rng.standard_normal(n).astype(np.float32)
+    w = rng.standard_normal(n).astype(np.float32)
+    dv, dw = to_dev(v, dev), to_dev(w, dev)
+    gc.check(gc.lib.gpucad_svl_axpby(gc.ptr(dv), 1.0, gc.ptr(dw), 0.37, n))
+    # fused a2 w + a1 v, computed exactly in double then rounded once
+    ref = (np.float64(np.float32(0.37)) * w + v).astype(np.float32)
+    assert np.array_equal(dv.cpu().numpy(), ref)
+    gc.check(gc.lib.gpucad_svl_fillfrac(gc.ptr(dv), gc.ptr(dw), n, 1, 1))
+    assert np.array_equal(dv.cpu().numpy(), (w * ref).astype(np.float32))
+    out = torch.zeros(1, dtype=torch.float32, device=dev)
+    gc.check(gc.lib.gpucad_svl_dot(gc.ptr(out), gc.ptr(dv), gc.ptr(dw), n))
+    exact = np.dot(dv.cpu().numpy().astype(np.float64), w.astype(np.float64))
+    assert abs(out.item() - exact) <= 1e-6 * abs(exact) + 1e-6
+
+
 def test_fft_fill_and_scale_bit_exact(gc, orc, dev):
     """The x = 0 plane is written twice (direct and mirrored, as in the
     reference kernel); a real field's spectrum is Hermitian there, so both
@@ -216,3 +234,39 @@ def test_svl_argument_errors(gc, dev):
     assert gc.lib.gpucad_svl_field(C.byref(p), None, gc.ptr(c), gc.ptr(out), C.byref(r),
                                    None) == gc.E_ARG
     assert gc.lib.gpucad_svl_matvec(gc.ptr(c), gc.ptr(out), 3, 4, 4) == gc.E_ARG
+
+
+def test_compat_spatial_lattice_loop(gc, dev, tmp_path):
+    """tests/cpp/svl_harness runs the reference caller's synthesis loop
+    (62 x finding_phi, GPUCG_lattice(500, 1, 0.01), texture, grating, svl,
+    normalise_four; then computeIsosurface_lattice and file_write_obj)
+    through the compat classes.  Each solve meets the reference's stopping
+    rule, the mesh is written, and the field agrees with the one-call
+    synthesis (both solve to a residual of 0.01 per harmonic, so they differ
+    by the solves' error, not by construction)."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "tests", "cpp", "svl_harness")
+    n, period = 16, 6.0
+    r = subprocess.run([exe, str(tmp_path), str(n), str(period), "b"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = [ln.split() for ln in r.stdout.strip().splitlines()]
+    per_h, last = lines[:-1], lines[-1]
+    assert len(per_h) == 62
+    for h, it, res in per_h:
+        assert int(it) == 500 or float(res) <= 0.01, (h, it, res)
+    active, total = int(last[0]), int(last[1])
+    assert active > 0 and total > 0 and total % 3 == 0
+    assert os.path.getsize(os.path.join(str(tmp_path), "spatial_lattice.obj")) > 0
+    coeffs = np.fromfile(os.path.join(str(tmp_path), "coeffs.bin"), np.float32)
+    loop = np.fromfile(os.path.join(str(tmp_path), "svl.bin"), np.float32)
+    p = gc.svl_params(n, lattice_type="b", uniform_type=0, const_period=period,
+                      cg_end_res=0.01, cg_max_iter=500)
+    one, res = gc.svl_field(p, to_dev(coeffs, dev))
+    one = one.cpu().numpy()
+    rms = np.sqrt(np.mean(loop.astype(np.float64) ** 2))
+    diff = np.sqrt(np.mean((one.astype(np.float64) - loop) ** 2))
+    assert diff < 0.05 * rms, (diff, rms)
+    assert np.corrcoef(one, loop)[0, 1] > 0.995
