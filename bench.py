@@ -17,6 +17,7 @@ Prints one JSON line:
               timed on this host on a bounded z-slab of the same workload
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -66,6 +67,8 @@ def parse():
                     help="also run a 512 MiB device copy (counter calibration)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched form")
     ap.add_argument("--no-serve", action="store_true", help="skip the served form")
+    ap.add_argument("--no-svl", action="store_true",
+                    help="skip the spatially variant lattice synthesis")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the 128^3 plain gyroid and 1024^3 bent diamond workloads")
     ap.add_argument("--batch-runs", type=int, default=10)
@@ -151,6 +154,73 @@ def bench_config(p, name, steps, warmup):
     del pipe
     torch.cuda.empty_cache()
     return out
+
+
+def bench_svl(n=120, period=30.0, lattice_type="b"):
+    """Spatially variant lattice synthesis at the reference's largest grid
+    (coarse n^3, fine (2n)^3; 61^3 unit cell, 62 of 125 coefficients):
+    the reference's per-harmonic loop (finding_phi, CG(500, 0.01), grating,
+    svl, normalise_four per harmonic; src/main.cu:3949-4033 without the
+    display extraction) through the drop-in entry points, against the
+    one-call synthesis (three unit-phase solves + one fused pass) followed
+    by one normalise_four."""
+    dev = "cuda"
+    cell = torch.empty(61 ** 3, dtype=torch.float32, device=dev)
+    gc.check(gc.lib.gpucad_create_lattice(gc.ptr(cell), 61, 61, 61, 61 ** 3, 0))
+    coeffs = gc.unit_spectrum(cell, 61, 2)
+    p = gc.svl_params(n, lattice_type=lattice_type, uniform_type=0, const_period=period,
+                      cg_end_res=0.01, cg_max_iter=500)
+    fn = p.fx * p.fy * p.fz
+    ind = torch.empty(fn, dtype=torch.float32, device=dev)
+    nrm = torch.empty(fn, dtype=torch.float32, device=dev)
+
+    def one_call():
+        svl, res = gc.svl_field(p, coeffs)
+        gc.check(gc.lib.gpucad_normalise_four(gc.ptr(svl), gc.ptr(ind), gc.ptr(nrm), fn, p.fx,
+                                              p.fy, p.fz, 0.2, 0.3))
+        torch.cuda.synchronize()
+        return res
+
+    one_call()
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        res = one_call()
+        times.append(time.perf_counter() - t0)
+    one_ms = 1e3 * min(times)
+
+    phi = torch.empty(n ** 3, dtype=torch.float32, device=dev)
+    grating = torch.empty(2 * fn, dtype=torch.float32, device=dev)
+    svl = torch.zeros(fn, dtype=torch.float32, device=dev)
+    it, r = C.c_int32(0), C.c_float(0)
+    iters = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for h in range(p.harmonics):
+        i, j, k = h % 5 - 2, (h // 5) % 5 - 2, h // 25 - 2
+        gc.check(gc.lib.gpucad_svl_finding_phi(gc.ptr(phi), None, n, n, n, i, j, k, 1.0, 1.0, 1.0,
+                                               lattice_type.encode(), 0, period, 30.0, 30.0, 30.0,
+                                               0.16, 0.6, 0))
+        gc.check(gc.lib.gpucad_svl_cg(gc.ptr(phi), n, n, n, 500, 0.01, C.byref(it), C.byref(r)))
+        iters.append(it.value)
+        gc.check(gc.lib.gpucad_svl_grating(gc.ptr(phi), n, n, n, gc.ptr(grating), p.fx, p.fy,
+                                           p.fz, 0.5, 0.5, 0.5))
+        gc.check(gc.lib.gpucad_svl_accumulate(gc.ptr(svl), gc.ptr(grating), fn, h,
+                                              gc.ptr(coeffs)))
+        gc.check(gc.lib.gpucad_normalise_four(gc.ptr(svl), gc.ptr(ind), gc.ptr(nrm), fn, p.fx,
+                                              p.fy, p.fz, 0.2, 0.3))
+    torch.cuda.synchronize()
+    loop_ms = 1e3 * (time.perf_counter() - t0)
+    return {"workload": "SVL synthesis, %s lattice, period %g, coarse %d^3, fine %d^3, "
+                        "61^3 unit cell (type 0), %d harmonics" % (lattice_type, period, n,
+                                                                   2 * n, p.harmonics),
+            "one_call_ms": round(one_ms, 2),
+            "one_call_cg_iters": list(res.cg_iters),
+            "loop_ms": round(loop_ms, 1),
+            "loop_cg_iters_mean": round(sum(iters) / len(iters), 1),
+            "loop_cg_iters_max": max(iters),
+            "speedup_vs_loop": round(loop_ms / one_ms, 1),
+            "fine_gvoxels_per_s": round(fn / (one_ms * 1e-3) / 1e9, 3)}
 
 
 def cpu_baseline(p, slab, threads, budget_s=12.0):
@@ -408,6 +478,9 @@ def main():
         print("bench: other single-grid workloads", file=sys.stderr, flush=True)
         rec["other_configs"] = [bench_config(CF.plain_gyroid(128), "plain_gyroid", 20, 3),
                                 bench_config(CF.bent_diamond(1024), "bent_diamond", 5, 1)]
+    if not args.no_svl:
+        print("bench: spatially variant lattice synthesis", file=sys.stderr, flush=True)
+        rec["svl"] = bench_svl()
     if not args.no_batch:
         print("bench: batched form", file=sys.stderr, flush=True)
         rec["batched"] = bench_batched(args.batch_runs)
