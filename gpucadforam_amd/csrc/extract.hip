@@ -785,24 +785,60 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
                 const bool on = e < n;
                 const uint32_t ent = on ? list[e] : 0u;
                 const uint32_t j = ent & 63, x = ent >> 6;
-                const uint32_t gx0 = gather32(ex0, j), gy0 = gather32(ey0, j), gz0 = gather32(ez0, j);
-                const uint32_t gx1 = gather32(ex1, j), gy1 = gather32(ey1, j), gz1 = gather32(ez1, j);
-                const uint32_t gx2 = gather32(ex2, j), gy2 = gather32(ey2, j), gz2 = gather32(ez2, j);
-                const uint32_t gx3 = gather32(ex3, j), gy3 = gather32(ey3, j), gz3 = gather32(ez3, j);
-                const uint32_t a0 = bit32(gather32(wa, j), x);
-                const uint32_t cy = gather32(carry, j);
-                // bits of point x+1 (the next half's point 0 when x = 31)
+                // row by row: each row's edge ids go to LDS as soon as they
+                // are known, so only one row's gathered halves are live
                 const bool last = x == 31;
                 const uint32_t xn = last ? 0u : x + 1;
-                const uint32_t sx0 = last ? (cy & 1u) : bit32(gx0, xn);
-                const uint32_t sy0 = last ? ((cy >> 1) & 1u) : bit32(gy0, xn);
-                const uint32_t sx1 = last ? ((cy >> 2) & 1u) : bit32(gx1, xn);
-                const uint32_t sy1 = last ? ((cy >> 3) & 1u) : bit32(gy1, xn);
-                const uint32_t sx2 = last ? ((cy >> 4) & 1u) : bit32(gx2, xn);
-                const uint32_t x0 = bit32(gx0, x), y0 = bit32(gy0, x), z0 = bit32(gz0, x);
-                const uint32_t x1 = bit32(gx1, x), y1 = bit32(gy1, x), z1 = bit32(gz1, x);
-                const uint32_t x2 = bit32(gx2, x), y2 = bit32(gy2, x), z2 = bit32(gz2, x);
-                const uint32_t x3 = bit32(gx3, x);
+                const uint32_t below = (1u << x) - 1u;
+                const uint32_t cy = gather32(carry, j);
+                const uint32_t a0 = bit32(gather32(wa, j), x);
+                uint32_t x0, y0, z0, x1, z1, x2, x3;
+                {  // row (y, z): x- and y-edges at x and x+1, z-edges at x and x+1
+                    const uint32_t gx = gather32(ex0, j), gy = gather32(ey0, j), gz = gather32(ez0, j);
+                    x0 = bit32(gx, x);
+                    y0 = bit32(gy, x);
+                    z0 = bit32(gz, x);
+                    const uint32_t sx = last ? (cy & 1u) : bit32(gx, xn);
+                    const uint32_t sy = last ? ((cy >> 1) & 1u) : bit32(gy, xn);
+                    const uint32_t p = gather32(vb0, j) + __popc(gx & below) + __popc(gy & below) +
+                                       __popc(gz & below);
+                    const uint32_t q = p + x0 + y0 + z0;  // first edge of point x+1
+                    vid[0 * 64 + (lane ^ 0)] = p;             // x-edge of (x,   y,   z)
+                    vid[3 * 64 + (lane ^ 3)] = p + x0;        // y-edge of (x,   y,   z)
+                    vid[8 * 64 + (lane ^ 8)] = p + x0 + y0;   // z-edge of (x,   y,   z)
+                    vid[1 * 64 + (lane ^ 1)] = q + sx;        // y-edge of (x+1, y,   z)
+                    vid[9 * 64 + (lane ^ 9)] = q + sx + sy;   // z-edge of (x+1, y,   z)
+                }
+                {  // row (y+1, z)
+                    const uint32_t gx = gather32(ex1, j), gy = gather32(ey1, j), gz = gather32(ez1, j);
+                    x1 = bit32(gx, x);
+                    const uint32_t y1 = bit32(gy, x);
+                    z1 = bit32(gz, x);
+                    const uint32_t sx = last ? ((cy >> 2) & 1u) : bit32(gx, xn);
+                    const uint32_t sy = last ? ((cy >> 3) & 1u) : bit32(gy, xn);
+                    const uint32_t p = gather32(vb1, j) + __popc(gx & below) + __popc(gy & below) +
+                                       __popc(gz & below);
+                    vid[2 * 64 + (lane ^ 2)] = p;                         // x-edge of (x,   y+1, z)
+                    vid[11 * 64 + (lane ^ 11)] = p + x1 + y1;             // z-edge of (x,   y+1, z)
+                    vid[10 * 64 + (lane ^ 10)] = p + x1 + y1 + z1 + sx + sy;  // z-edge of (x+1, y+1, z)
+                }
+                {  // row (y, z+1)
+                    const uint32_t gx = gather32(ex2, j), gy = gather32(ey2, j), gz = gather32(ez2, j);
+                    x2 = bit32(gx, x);
+                    const uint32_t y2 = bit32(gy, x), z2 = bit32(gz, x);
+                    const uint32_t sx = last ? ((cy >> 4) & 1u) : bit32(gx, xn);
+                    const uint32_t p = gather32(vb2, j) + __popc(gx & below) + __popc(gy & below) +
+                                       __popc(gz & below);
+                    vid[4 * 64 + (lane ^ 4)] = p;                    // x-edge of (x,   y,   z+1)
+                    vid[7 * 64 + (lane ^ 7)] = p + x2;               // y-edge of (x,   y,   z+1)
+                    vid[5 * 64 + (lane ^ 5)] = p + x2 + y2 + z2 + sx;  // y-edge of (x+1, y,   z+1)
+                }
+                {  // row (y+1, z+1)
+                    const uint32_t gx = gather32(ex3, j), gy = gather32(ey3, j), gz = gather32(ez3, j);
+                    x3 = bit32(gx, x);
+                    vid[6 * 64 + (lane ^ 6)] = gather32(vb3, j) + __popc(gx & below) +
+                                               __popc(gy & below) + __popc(gz & below);
+                }
                 // corners: 0 (x,y,z) 1 (x+1,y,z) 2 (x+1,y+1,z) 3 (x,y+1,z)
                 //          4 (x,y,z+1) 5 (x+1,y,z+1) 6 (x+1,y+1,z+1) 7 (x,y+1,z+1)
                 const uint32_t k0 = a0, k1 = a0 ^ x0, k3 = a0 ^ y0, k2 = k3 ^ x1;
@@ -813,31 +849,7 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
                 const uint32_t nt = on ? (uint32_t)(packed >> 60) : 0u;
                 const uint32_t tincl = wave_scan_incl(nt);
                 const uint32_t tex = tincl - nt, ttot = readlane32(tincl, 63);
-                const uint32_t below = (1u << x) - 1u;
-                const uint32_t p0 = gather32(vb0, j) + __popc(gx0 & below) + __popc(gy0 & below) +
-                                    __popc(gz0 & below);
-                const uint32_t p1 = gather32(vb1, j) + __popc(gx1 & below) + __popc(gy1 & below) +
-                                    __popc(gz1 & below);
-                const uint32_t p2 = gather32(vb2, j) + __popc(gx2 & below) + __popc(gy2 & below) +
-                                    __popc(gz2 & below);
-                const uint32_t p3 = gather32(vb3, j) + __popc(gx3 & below) + __popc(gy3 & below) +
-                                    __popc(gz3 & below);
-                const uint32_t q0 = p0 + x0 + y0 + z0;  // first edge of point x+1
-                const uint32_t q1 = p1 + x1 + y1 + z1;
-                const uint32_t q2 = p2 + x2 + y2 + z2;
                 if (on) {
-                    vid[0 * 64 + (lane ^ 0)] = p0;              // x-edge of (x,   y,   z)
-                    vid[1 * 64 + (lane ^ 1)] = q0 + sx0;        // y-edge of (x+1, y,   z)
-                    vid[2 * 64 + (lane ^ 2)] = p1;              // x-edge of (x,   y+1, z)
-                    vid[3 * 64 + (lane ^ 3)] = p0 + x0;         // y-edge of (x,   y,   z)
-                    vid[4 * 64 + (lane ^ 4)] = p2;              // x-edge of (x,   y,   z+1)
-                    vid[5 * 64 + (lane ^ 5)] = q2 + sx2;        // y-edge of (x+1, y,   z+1)
-                    vid[6 * 64 + (lane ^ 6)] = p3;              // x-edge of (x,   y+1, z+1)
-                    vid[7 * 64 + (lane ^ 7)] = p2 + x2;         // y-edge of (x,   y,   z+1)
-                    vid[8 * 64 + (lane ^ 8)] = p0 + x0 + y0;    // z-edge of (x,   y,   z)
-                    vid[9 * 64 + (lane ^ 9)] = q0 + sx0 + sy0;  // z-edge of (x+1, y,   z)
-                    vid[10 * 64 + (lane ^ 10)] = q1 + sx1 + sy1; // z-edge of (x+1, y+1, z)
-                    vid[11 * 64 + (lane ^ 11)] = p1 + x1 + y1;   // z-edge of (x,   y+1, z)
                     pk[lane] = packed;
                     for (uint32_t jt = 0; jt < nt; ++jt) own[tex + jt] = (uint16_t)(lane | (jt << 8));
                 }
