@@ -24,27 +24,38 @@
 namespace gc {
 namespace {
 
-template <bool ROT, int TPMS, bool THICK>
+template <bool ROT, int TPMS, bool THICK, int PRIM>
 __global__ void __launch_bounds__(64 * kFieldWaves)
 lattice_field_kernel(FieldConsts c, float *__restrict__ field, uint64_t *__restrict__ signs) {
     __shared__ float ztab[kMaxLZ * kZTab];
-    lattice_brick<ROT, TPMS, THICK, 64>(c, field, signs, blockIdx.x * 64, blockIdx.y * kFieldWaves,
-                                        blockIdx.z * c.lz, ztab);
+    lattice_brick<ROT, TPMS, THICK, 64, PRIM>(c, field, signs, blockIdx.x * 64,
+                                              blockIdx.y * kFieldWaves, blockIdx.z * c.lz, ztab);
 }
 
 typedef void (*FieldKernel)(FieldConsts, float *, uint64_t *);
 
+// the primitive is a template parameter of the one-grid kernel: the per-point
+// loop then carries no branch on it
+template <bool ROT, int TPMS, bool THICK>
+FieldKernel pick_prim(int prim) {
+    switch (prim) {
+        case GC_PRIM_SPHERE: return lattice_field_kernel<ROT, TPMS, THICK, GC_PRIM_SPHERE>;
+        case GC_PRIM_BOX: return lattice_field_kernel<ROT, TPMS, THICK, GC_PRIM_BOX>;
+        default: return lattice_field_kernel<ROT, TPMS, THICK, GC_PRIM_NONE>;
+    }
+}
+
 template <bool ROT, int TPMS>
-FieldKernel pick_thick(bool thick) {
-    return thick ? lattice_field_kernel<ROT, TPMS, true> : lattice_field_kernel<ROT, TPMS, false>;
+FieldKernel pick_thick(bool thick, int prim) {
+    return thick ? pick_prim<ROT, TPMS, true>(prim) : pick_prim<ROT, TPMS, false>(prim);
 }
 
 template <bool ROT>
-FieldKernel pick_tpms(int tpms, bool thick) {
+FieldKernel pick_tpms(int tpms, bool thick, int prim) {
     switch (tpms) {
-        case GC_TPMS_GYROID: return pick_thick<ROT, GC_TPMS_GYROID>(thick);
-        case GC_TPMS_SCHWARZ_P: return pick_thick<ROT, GC_TPMS_SCHWARZ_P>(thick);
-        default: return pick_thick<ROT, GC_TPMS_DIAMOND>(thick);
+        case GC_TPMS_GYROID: return pick_thick<ROT, GC_TPMS_GYROID>(thick, prim);
+        case GC_TPMS_SCHWARZ_P: return pick_thick<ROT, GC_TPMS_SCHWARZ_P>(thick, prim);
+        default: return pick_thick<ROT, GC_TPMS_DIAMOND>(thick, prim);
     }
 }
 
@@ -80,7 +91,8 @@ int launch_lattice_field(const gc_lattice_params *p, float *d_field, uint64_t *d
     const FieldConsts c = make_field_consts(p, iso, lz);
     const bool rot = p->rot0 != 0.f || p->rot1 != 0.f;
     const bool thick = p->n_modes > 0;
-    FieldKernel k = rot ? pick_tpms<true>(p->tpms, thick) : pick_tpms<false>(p->tpms, thick);
+    FieldKernel k = rot ? pick_tpms<true>(p->tpms, thick, c.prim)
+                        : pick_tpms<false>(p->tpms, thick, c.prim);
     dim3 grid((p->nx + 63) / 64, (p->ny + kFieldWaves - 1) / kFieldWaves, (p->nz + lz - 1) / lz);
     GC_ARG(grid.y <= 65535 && grid.z <= 65535, "grid too large for one launch");
     hipLaunchKernelGGL(k, grid, dim3(64 * kFieldWaves), 0, stream, c, d_field, d_signs);

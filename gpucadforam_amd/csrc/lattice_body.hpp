@@ -157,11 +157,18 @@ __device__ __forceinline__ void store_sign_bits(uint64_t *row, int x0, uint64_t 
     }
 }
 
+// Primitive of a lattice field known at compile time (PRIM_ANY: read from
+// the constants at run time).
+constexpr int PRIM_ANY = -1;
+
 // Evaluates the brick with first point (x0, y0, z0) of the grid `c`.  Must be
 // called by all 256 threads of the workgroup (it synchronises through LDS);
 // `ztab` is kMaxLZ * kZTab floats of LDS.  `field` and `signs` point at the
-// grid's own first point and sign word; `signs` may be NULL.
-template <bool ROT, int TPMS, bool THICK, int BX>
+// grid's own first point and sign word; `signs` may be NULL.  PRIM is c.prim
+// or PRIM_ANY.  The per-point loop has no data-independent branches: absent
+// thickness groups have zero coefficients (their fmas add exactly 0) and the
+// boolean is a select among max(g, s), min(g, s), max(g, -s).
+template <bool ROT, int TPMS, bool THICK, int BX, int PRIM = PRIM_ANY>
 __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__restrict__ field,
                                               uint64_t *__restrict__ signs, int x0, int y0,
                                               int z0, float *ztab) {
@@ -332,33 +339,31 @@ __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__res
         }
         float t = T0;
         if (THICK) {
-            if (c.group_mask & 2) {
-                t = __builtin_fmaf(S1.re, e[ZT_E1 + 0], t);
-                t = __builtin_fmaf(-S1.im, e[ZT_E1 + 1], t);
-            }
-            if (c.group_mask & 4) {
-                t = __builtin_fmaf(S2.re, e[ZT_E1 + 2], t);
-                t = __builtin_fmaf(-S2.im, e[ZT_E1 + 3], t);
-            }
-            if (c.group_mask & 8) {
-                t = __builtin_fmaf(S3.re, e[ZT_E1 + 4], t);
-                t = __builtin_fmaf(-S3.im, e[ZT_E1 + 5], t);
-            }
+            // a group without modes has S = 0: its fmas add exactly 0
+            t = __builtin_fmaf(S1.re, e[ZT_E1 + 0], t);
+            t = __builtin_fmaf(-S1.im, e[ZT_E1 + 1], t);
+            t = __builtin_fmaf(S2.re, e[ZT_E1 + 2], t);
+            t = __builtin_fmaf(-S2.im, e[ZT_E1 + 3], t);
+            t = __builtin_fmaf(S3.re, e[ZT_E1 + 4], t);
+            t = __builtin_fmaf(-S3.im, e[ZT_E1 + 5], t);
         }
         float g = c.sheet ? __builtin_fabsf(f) - t : f;
-        if (c.prim != GC_PRIM_NONE) {
+        const int prim = PRIM == PRIM_ANY ? c.prim : PRIM;
+        if (prim != GC_PRIM_NONE) {
             float s;
             const float pz = e[ZT_PRIMZ];
-            if (c.prim == GC_PRIM_SPHERE) {
+            if (prim == GC_PRIM_SPHERE) {
                 s = __builtin_sqrtf(__builtin_fmaf(pz, pz, pxy)) - c.pr;
             } else {
                 float mz = __builtin_fmaxf(pz, 0.f);
                 s = __builtin_sqrtf(__builtin_fmaf(mz, mz, pxy)) +
                     __builtin_fminf(__builtin_fmaxf(pbxy, pz), 0.f);
             }
-            if (c.op == GC_OP_INTERSECT) g = __builtin_fmaxf(g, s);
-            else if (c.op == GC_OP_UNION) g = __builtin_fminf(g, s);
-            else if (c.op == GC_OP_SUBTRACT) g = __builtin_fmaxf(g, -s);
+            const float gi = __builtin_fmaxf(g, s), gu = __builtin_fminf(g, s),
+                        gs = __builtin_fmaxf(g, -s);
+            g = c.op == GC_OP_INTERSECT ? gi
+                                        : (c.op == GC_OP_UNION ? gu
+                                                               : (c.op == GC_OP_SUBTRACT ? gs : g));
         }
         if (xin && yin) out[(uint64_t)i * plane] = g;
         if (srow) {
