@@ -24,38 +24,49 @@
 namespace gc {
 namespace {
 
-template <bool ROT, int TPMS, bool THICK, int PRIM>
+template <bool ROT, int TPMS, bool THICK, int PRIM, int OP>
 __global__ void __launch_bounds__(64 * kFieldWaves)
 lattice_field_kernel(FieldConsts c, float *__restrict__ field, uint64_t *__restrict__ signs) {
     __shared__ float ztab[kMaxLZ * kZTab];
-    lattice_brick<ROT, TPMS, THICK, 64, PRIM>(c, field, signs, blockIdx.x * 64,
-                                              blockIdx.y * kFieldWaves, blockIdx.z * c.lz, ztab);
+    lattice_brick<ROT, TPMS, THICK, 64, PRIM, OP>(c, field, signs, blockIdx.x * 64,
+                                                  blockIdx.y * kFieldWaves, blockIdx.z * c.lz,
+                                                  ztab);
 }
 
 typedef void (*FieldKernel)(FieldConsts, float *, uint64_t *);
 
-// the primitive is a template parameter of the one-grid kernel: the per-point
-// loop then carries no branch on it
+// the primitive and the boolean are template parameters of the one-grid
+// kernel: the per-point loop then carries no branch on them
+template <bool ROT, int TPMS, bool THICK, int PRIM>
+FieldKernel pick_op(int op) {
+    switch (op) {
+        case GC_OP_UNION: return lattice_field_kernel<ROT, TPMS, THICK, PRIM, GC_OP_UNION>;
+        case GC_OP_SUBTRACT: return lattice_field_kernel<ROT, TPMS, THICK, PRIM, GC_OP_SUBTRACT>;
+        case GC_OP_INTERSECT: return lattice_field_kernel<ROT, TPMS, THICK, PRIM, GC_OP_INTERSECT>;
+        default: return lattice_field_kernel<ROT, TPMS, THICK, PRIM, GC_OP_NONE>;
+    }
+}
+
 template <bool ROT, int TPMS, bool THICK>
-FieldKernel pick_prim(int prim) {
+FieldKernel pick_prim(int prim, int op) {
     switch (prim) {
-        case GC_PRIM_SPHERE: return lattice_field_kernel<ROT, TPMS, THICK, GC_PRIM_SPHERE>;
-        case GC_PRIM_BOX: return lattice_field_kernel<ROT, TPMS, THICK, GC_PRIM_BOX>;
-        default: return lattice_field_kernel<ROT, TPMS, THICK, GC_PRIM_NONE>;
+        case GC_PRIM_SPHERE: return pick_op<ROT, TPMS, THICK, GC_PRIM_SPHERE>(op);
+        case GC_PRIM_BOX: return pick_op<ROT, TPMS, THICK, GC_PRIM_BOX>(op);
+        default: return lattice_field_kernel<ROT, TPMS, THICK, GC_PRIM_NONE, GC_OP_NONE>;
     }
 }
 
 template <bool ROT, int TPMS>
-FieldKernel pick_thick(bool thick, int prim) {
-    return thick ? pick_prim<ROT, TPMS, true>(prim) : pick_prim<ROT, TPMS, false>(prim);
+FieldKernel pick_thick(bool thick, int prim, int op) {
+    return thick ? pick_prim<ROT, TPMS, true>(prim, op) : pick_prim<ROT, TPMS, false>(prim, op);
 }
 
 template <bool ROT>
-FieldKernel pick_tpms(int tpms, bool thick, int prim) {
+FieldKernel pick_tpms(int tpms, bool thick, int prim, int op) {
     switch (tpms) {
-        case GC_TPMS_GYROID: return pick_thick<ROT, GC_TPMS_GYROID>(thick, prim);
-        case GC_TPMS_SCHWARZ_P: return pick_thick<ROT, GC_TPMS_SCHWARZ_P>(thick, prim);
-        default: return pick_thick<ROT, GC_TPMS_DIAMOND>(thick, prim);
+        case GC_TPMS_GYROID: return pick_thick<ROT, GC_TPMS_GYROID>(thick, prim, op);
+        case GC_TPMS_SCHWARZ_P: return pick_thick<ROT, GC_TPMS_SCHWARZ_P>(thick, prim, op);
+        default: return pick_thick<ROT, GC_TPMS_DIAMOND>(thick, prim, op);
     }
 }
 
@@ -91,8 +102,8 @@ int launch_lattice_field(const gc_lattice_params *p, float *d_field, uint64_t *d
     const FieldConsts c = make_field_consts(p, iso, lz);
     const bool rot = p->rot0 != 0.f || p->rot1 != 0.f;
     const bool thick = p->n_modes > 0;
-    FieldKernel k = rot ? pick_tpms<true>(p->tpms, thick, c.prim)
-                        : pick_tpms<false>(p->tpms, thick, c.prim);
+    FieldKernel k = rot ? pick_tpms<true>(p->tpms, thick, c.prim, c.op)
+                        : pick_tpms<false>(p->tpms, thick, c.prim, c.op);
     dim3 grid((p->nx + 63) / 64, (p->ny + kFieldWaves - 1) / kFieldWaves, (p->nz + lz - 1) / lz);
     GC_ARG(grid.y <= 65535 && grid.z <= 65535, "grid too large for one launch");
     hipLaunchKernelGGL(k, grid, dim3(64 * kFieldWaves), 0, stream, c, d_field, d_signs);

@@ -164,11 +164,11 @@ constexpr int PRIM_ANY = -1;
 // Evaluates the brick with first point (x0, y0, z0) of the grid `c`.  Must be
 // called by all 256 threads of the workgroup (it synchronises through LDS);
 // `ztab` is kMaxLZ * kZTab floats of LDS.  `field` and `signs` point at the
-// grid's own first point and sign word; `signs` may be NULL.  PRIM is c.prim
-// or PRIM_ANY.  The per-point loop has no data-independent branches: absent
+// grid's own first point and sign word; `signs` may be NULL.  PRIM and OP are
+// c.prim and c.op, or PRIM_ANY (read at run time).  The per-point loop has no data-independent branches: absent
 // thickness groups have zero coefficients (their fmas add exactly 0) and the
 // boolean is a select among max(g, s), min(g, s), max(g, -s).
-template <bool ROT, int TPMS, bool THICK, int BX, int PRIM = PRIM_ANY>
+template <bool ROT, int TPMS, bool THICK, int BX, int PRIM = PRIM_ANY, int OP = PRIM_ANY>
 __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__restrict__ field,
                                               uint64_t *__restrict__ signs, int x0, int y0,
                                               int z0, float *ztab) {
@@ -359,11 +359,11 @@ __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__res
                 s = __builtin_sqrtf(__builtin_fmaf(mz, mz, pxy)) +
                     __builtin_fminf(__builtin_fmaxf(pbxy, pz), 0.f);
             }
+            const int op = OP == PRIM_ANY ? c.op : OP;
             const float gi = __builtin_fmaxf(g, s), gu = __builtin_fminf(g, s),
                         gs = __builtin_fmaxf(g, -s);
-            g = c.op == GC_OP_INTERSECT ? gi
-                                        : (c.op == GC_OP_UNION ? gu
-                                                               : (c.op == GC_OP_SUBTRACT ? gs : g));
+            g = op == GC_OP_INTERSECT ? gi
+                                      : (op == GC_OP_UNION ? gu : (op == GC_OP_SUBTRACT ? gs : g));
         }
         if (xin && yin) out[(uint64_t)i * plane] = g;
         if (srow) {
