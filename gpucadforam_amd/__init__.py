@@ -299,6 +299,13 @@ SIGNATURES = {
     "gpucad_unit_spectrum": ([_vp, _i32, _i32, _i32, _i32, _vp], C.c_int),
     "gpucad_svl_field": ([C.POINTER(SvlParams), _vp, _vp, _vp, C.POINTER(SvlResult), _vp],
                          C.c_int),
+    "gpucad_import_external_buffer": ([_i32, _u64, _u64, C.POINTER(_vp), C.POINTER(_vp)],
+                                      C.c_int),
+    "gpucad_release_external_buffer": ([_vp], C.c_int),
+    "gpucad_import_external_semaphore": ([_i32, C.POINTER(_vp)], C.c_int),
+    "gpucad_signal_external_semaphore": ([_vp, _vp], C.c_int),
+    "gpucad_wait_external_semaphore": ([_vp, _vp], C.c_int),
+    "gpucad_release_external_semaphore": ([_vp], C.c_int),
     "gpucad_file_write_obj": ([_vp, _u32, C.c_char_p], C.c_int),
     "gpucad_write_obj_indexed": ([_vp, _u64, _vp, _u64, C.c_char_p, _i32], C.c_int),
 }
@@ -631,3 +638,4 @@ def svl_field(params, coeffs, period=None, stream=None):
     check(lib.gpucad_svl_field(C.byref(params), ptr(period), ptr(coeffs), ptr(out), C.byref(res),
                                stream_handle(stream)))
     return out, res
+

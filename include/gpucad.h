@@ -622,6 +622,30 @@ int gpucad_compute_isosurface(gc_float4 *pos, gc_float4 *norm, float iso, uint32
                               float iso1, float iso2, uint32_t flags, uint64_t *nfacets);
 
 /* ------------------------------------------------------------------------ */
+/* Graphics-API interop (pos/norm buffers and semaphores shared with Vulkan)  */
+/* ------------------------------------------------------------------------ */
+typedef struct gc_external_buffer gc_external_buffer;       /* opaque */
+typedef struct gc_external_semaphore gc_external_semaphore; /* opaque */
+/* importCudaExternalMemory (src/main.cu:1164-1191): imports the allocation of
+ * `size` bytes exported as an opaque POSIX fd (vkGetMemoryFdKHR) and maps
+ * bytes [offset, size) as a device buffer (*d_ptr; the reference maps offset
+ * 0).  As with the CUDA call, a successful import owns the fd.  The extraction
+ * entry points write into *d_ptr in place. */
+int gpucad_import_external_buffer(int32_t fd, uint64_t size, uint64_t offset,
+                                  gc_external_buffer **buf, void **d_ptr);
+/* cudaDestroyExternalMemory (src/main.cu:754-931): unmaps and releases. */
+int gpucad_release_external_buffer(gc_external_buffer *buf);
+/* importCudaExternalSemaphore (src/main.cu:1193-1211): binary semaphore
+ * exported as an opaque fd. */
+int gpucad_import_external_semaphore(int32_t fd, gc_external_semaphore **sem);
+/* cudaSignalExternalSemaphoresAsync / cudaWaitExternalSemaphoresAsync with
+ * fence value 0 (src/main.cu:2017-2035, 3122-3134), enqueued on `stream`. */
+int gpucad_signal_external_semaphore(gc_external_semaphore *sem, void *stream);
+int gpucad_wait_external_semaphore(gc_external_semaphore *sem, void *stream);
+/* cudaDestroyExternalSemaphore (src/main.cu:739-744). */
+int gpucad_release_external_semaphore(gc_external_semaphore *sem);
+
+/* ------------------------------------------------------------------------ */
 /* .obj output                                                               */
 /* ------------------------------------------------------------------------ */
 /* File_output::file_write_obj (src/File_output.h:189, src/File_output.cu:5-81):

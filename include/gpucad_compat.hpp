@@ -550,3 +550,36 @@ public:
                              "file_write_obj");
     }
 };
+
+// The caller's Vulkan interop helpers (src/main.cu:1164-1211, 2017-2035,
+// 739-931) with the CUDA runtime calls replaced: the caller still obtains the
+// fd from Vulkan (vkGetMemoryFdKHR / vkGetSemaphoreFdKHR) and passes it in.
+// Errors abort like checkCudaErrors.
+namespace gpucad_interop {
+
+// importCudaExternalMemory: the whole allocation of `size` bytes, offset 0.
+inline void importExternalMemory(void **devPtr, gc_external_buffer *&mem, int fd,
+                                 unsigned long long size) {
+    gpucad_compat::check(gpucad_import_external_buffer(fd, size, 0, &mem, devPtr),
+                         "importExternalMemory");
+}
+inline void destroyExternalMemory(gc_external_buffer *mem) {
+    gpucad_compat::check(gpucad_release_external_buffer(mem), "destroyExternalMemory");
+}
+// importCudaExternalSemaphore (binary semaphore, opaque fd).
+inline void importExternalSemaphore(gc_external_semaphore *&sem, int fd) {
+    gpucad_compat::check(gpucad_import_external_semaphore(fd, &sem), "importExternalSemaphore");
+}
+// cudaSignalExternalSemaphoresAsync / cudaWaitExternalSemaphoresAsync with
+// fence value 0 on the default stream, as drawFrame does.
+inline void signalExternalSemaphore(gc_external_semaphore *sem) {
+    gpucad_compat::check(gpucad_signal_external_semaphore(sem, nullptr), "signalExternalSemaphore");
+}
+inline void waitExternalSemaphore(gc_external_semaphore *sem) {
+    gpucad_compat::check(gpucad_wait_external_semaphore(sem, nullptr), "waitExternalSemaphore");
+}
+inline void destroyExternalSemaphore(gc_external_semaphore *sem) {
+    gpucad_compat::check(gpucad_release_external_semaphore(sem), "destroyExternalSemaphore");
+}
+
+}  // namespace gpucad_interop
