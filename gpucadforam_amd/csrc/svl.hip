@@ -37,12 +37,39 @@ constexpr double kTwoPi = 6.283185307179586476925286766559;
 __device__ __forceinline__ float cos_rad(float t) { return cos_rev(t * kInv2Pi); }
 __device__ __forceinline__ float sin_rad(float t) { return sin_rev(t * kInv2Pi); }
 
+// (x, y, z) of linear point t; grids have fewer than 2^32 points (checked
+// at the entry points), so the divisions are 32-bit
 __device__ __forceinline__ void point_xyz(uint64_t t, int nx, int ny, int *x, int *y, int *z) {
-    const uint64_t plane = (uint64_t)nx * ny;
-    *z = (int)(t / plane);
-    const uint64_t r = t - (uint64_t)*z * plane;
-    *y = (int)(r / (uint64_t)nx);
-    *x = (int)(r - (uint64_t)*y * nx);
+    const uint32_t tt = (uint32_t)t, plane = (uint32_t)nx * (uint32_t)ny;
+    const uint32_t zz = tt / plane;
+    const uint32_t r = tt - zz * plane;
+    const uint32_t yy = r / (uint32_t)nx;
+    *z = (int)zz;
+    *y = (int)yy;
+    *x = (int)(r - yy * (uint32_t)nx);
+}
+
+// n / d for n < 2^32 through a double reciprocal and one correction step
+struct Div32 {
+    uint32_t d;
+    double inv;
+};
+inline Div32 make_div32(uint32_t d) { return Div32{d, 1.0 / (double)d}; }
+__device__ __forceinline__ uint32_t div32(uint32_t n, const Div32 &f) {
+    uint32_t q = (uint32_t)((double)n * f.inv);
+    const int64_t r = (int64_t)n - (int64_t)q * f.d;
+    if (r < 0) --q;
+    else if (r >= (int64_t)f.d) ++q;
+    return q;
+}
+__device__ __forceinline__ void point_xyz(uint32_t t, const Div32 &dnx, const Div32 &dplane,
+                                          int *x, int *y, int *z) {
+    const uint32_t zz = div32(t, dplane);
+    const uint32_t r = t - zz * dplane.d;
+    const uint32_t yy = div32(r, dnx);
+    *z = (int)zz;
+    *y = (int)yy;
+    *x = (int)(r - yy * dnx.d);
 }
 
 // ---------------------------------------------------------------------------
@@ -207,20 +234,26 @@ __global__ void finding_phi_kernel(float *phi, const float *period, PhiArgs p) {
 // the operator D^T D of the phase solve (GPUMatvec_lattice_kernel,
 // Gratings.cu:420-597): per axis a diagonal weight and two neighbour terms
 // ---------------------------------------------------------------------------
+// Coordinate c of n (n >= 4) along stride s:
+//   c = 0:     w 1.25, t2 = -d(t+s),         t3 = -0.25 d(t+2s)
+//   c = 1:     w 1.25, t2 = -d(t-s),         t3 = -0.25 d(t+2s)
+//   c = n-2:   w 1.25, t2 = -0.25 d(t-2s),   t3 = -d(t+s)
+//   c = n-1:   w 1.25, t2 = -0.25 d(t-2s),   t3 = -d(t-s)
+//   otherwise: w 0.5,  t2 = -0.25 d(t-2s),   t3 = -0.25 d(t+2s)
+// chosen by selects, so both loads are unconditional (every point's loads
+// can be in flight together); -1 * v and -0.25 * v are the exact negation
+// and quarter the cases name.
 template <class G>
 __device__ __forceinline__ void axis_terms(const G &get, int c, int n, int64_t t, int64_t s,
                                            float *w, float *t2, float *t3) {
-    if (c == 0) {
-        *w = 1.25f; *t2 = -get(t + s); *t3 = -0.25f * get(t + 2 * s);
-    } else if (c == 1) {
-        *w = 1.25f; *t2 = -get(t - s); *t3 = -0.25f * get(t + 2 * s);
-    } else if (c == n - 2) {
-        *w = 1.25f; *t2 = -0.25f * get(t - 2 * s); *t3 = -get(t + s);
-    } else if (c == n - 1) {
-        *w = 1.25f; *t2 = -0.25f * get(t - 2 * s); *t3 = -get(t - s);
-    } else {
-        *w = 0.5f; *t2 = -0.25f * get(t - 2 * s); *t3 = -0.25f * get(t + 2 * s);
-    }
+    const bool lo = c <= 1, hi1 = c == n - 2, hi0 = c == n - 1;
+    *w = (lo || hi1 || hi0) ? 1.25f : 0.5f;
+    const int64_t o2 = c == 0 ? s : (c == 1 ? -s : -2 * s);
+    const float k2 = lo ? -1.0f : -0.25f;
+    const int64_t o3 = lo ? 2 * s : (hi1 ? s : (hi0 ? -s : 2 * s));
+    const float k3 = lo ? -0.25f : ((hi1 || hi0) ? -1.0f : -0.25f);
+    *t2 = k2 * get(t + o2);
+    *t3 = k3 * get(t + o3);
 }
 
 // (wx + wy + wz) * d + x2 + x3 + y2 + y3 + z2 + z3 in this order, the first
@@ -269,9 +302,12 @@ struct CgState {
     int pad[3];
 };
 
+constexpr int kCgUnroll = 4;  // points per thread and step, all their loads in flight
+
 struct CgArgs {
     int nx, ny, nz, G;
     uint64_t n;
+    Div32 dnx, dplane;
     float *phi, *r, *q, *d0, *d1;  // m vectors of n each (right-hand side s at s*n)
     double *pq, *prr;              // partial sums: pq[m][G], prr[2][m][G]
     CgState *st;
@@ -340,23 +376,38 @@ __global__ void cg_a_kernel(CgArgs a, int it) {
     }
     const float beta = it == 0 ? 0.0f : delta / a.st[s].delta[(it + 1) & 1];
     if (blockIdx.x == 0 && threadIdx.x == 0) a.st[s].delta[it & 1] = delta;
-    const float *dprev = ((it - 1) & 1 ? a.d1 : a.d0) + m_off;
     float *dcur = (it & 1 ? a.d1 : a.d0) + m_off;
     const float *r = a.r + m_off;
     const bool first = it == 0;
-    auto get = [=](int64_t i) {
-        return first ? dcur[i] : __builtin_fmaf(beta, dprev[i], r[i]);
-    };
+    // d = r + beta d_prev; the first direction is r itself (fma(0, r, r) = r
+    // exactly), so one load pattern serves every iteration
+    const float *dprev = first ? r : ((it - 1) & 1 ? a.d1 : a.d0) + m_off;
+    const float bt = first ? 0.0f : beta;
+    auto get = [=](int64_t i) { return __builtin_fmaf(bt, dprev[i], r[i]); };
+    // points t0 + u * S (u < kCgUnroll) per step: every load of the step is
+    // issued before its stores; the partial sum adds the points in the order
+    // t0, t0 + S, t0 + 2S, ... as a one-point-per-step loop would
+    const uint32_t S = (uint32_t)a.G * kCgThreads;
     double acc = 0.0;
-    for (uint64_t t = (uint64_t)blockIdx.x * kCgThreads + threadIdx.x; t < a.n;
-         t += (uint64_t)a.G * kCgThreads) {
-        int x, y, z;
-        point_xyz(t, a.nx, a.ny, &x, &y, &z);
-        const float dv = get((int64_t)t);
-        const float qv = apply_op(get, x, y, z, a.nx, a.ny, a.nz, (int64_t)t);
-        if (!first) dcur[t] = dv;
-        a.q[m_off + t] = qv;
-        acc += (double)dv * (double)qv;
+    for (uint64_t t0 = blockIdx.x * kCgThreads + threadIdx.x; t0 < a.n; t0 += kCgUnroll * S) {
+        float dv[kCgUnroll], qv[kCgUnroll];
+#pragma unroll
+        for (int u = 0; u < kCgUnroll; ++u) {
+            const uint32_t t = (uint32_t)(t0 + u * S < a.n ? t0 + u * S : t0);
+            int x, y, z;
+            point_xyz(t, a.dnx, a.dplane, &x, &y, &z);
+            dv[u] = get((int64_t)t);
+            qv[u] = apply_op(get, x, y, z, a.nx, a.ny, a.nz, (int64_t)t);
+        }
+#pragma unroll
+        for (int u = 0; u < kCgUnroll; ++u) {
+            const uint64_t t = t0 + u * S;
+            if (t < a.n) {
+                if (!first) dcur[t] = dv[u];
+                a.q[m_off + t] = qv[u];
+                acc += (double)dv[u] * (double)qv[u];
+            }
+        }
     }
     const double tot = block_sum(acc);
     if (threadIdx.x == 0) a.pq[(uint64_t)s * a.G + blockIdx.x] = tot;
@@ -374,13 +425,28 @@ __global__ void cg_b_kernel(CgArgs a, int it) {
     const float *d = (it & 1 ? a.d1 : a.d0) + m_off;
     float *phi = a.phi + m_off, *r = a.r + m_off;
     const float *q = a.q + m_off;
+    const uint32_t S = (uint32_t)a.G * kCgThreads;
     double acc = 0.0;
-    for (uint64_t t = (uint64_t)blockIdx.x * kCgThreads + threadIdx.x; t < a.n;
-         t += (uint64_t)a.G * kCgThreads) {
-        phi[t] = __builtin_fmaf(alpha, d[t], phi[t]);
-        const float rv = __builtin_fmaf(-alpha, q[t], r[t]);
-        r[t] = rv;
-        acc += (double)rv * (double)rv;
+    for (uint64_t t0 = blockIdx.x * kCgThreads + threadIdx.x; t0 < a.n; t0 += kCgUnroll * S) {
+        float dv[kCgUnroll], pv[kCgUnroll], qv[kCgUnroll], rv[kCgUnroll];
+#pragma unroll
+        for (int u = 0; u < kCgUnroll; ++u) {
+            const uint32_t t = (uint32_t)(t0 + u * S < a.n ? t0 + u * S : t0);
+            dv[u] = d[t];
+            pv[u] = phi[t];
+            qv[u] = q[t];
+            rv[u] = r[t];
+        }
+#pragma unroll
+        for (int u = 0; u < kCgUnroll; ++u) {
+            const uint64_t t = t0 + u * S;
+            if (t < a.n) {
+                phi[t] = __builtin_fmaf(alpha, dv[u], pv[u]);
+                const float rn = __builtin_fmaf(-alpha, qv[u], rv[u]);
+                r[t] = rn;
+                acc += (double)rn * (double)rn;
+            }
+        }
     }
     const double tot = block_sum(acc);
     if (threadIdx.x == 0) a.prr[((uint64_t)((it + 1) & 1) * m + s) * a.G + blockIdx.x] = tot;
@@ -397,6 +463,8 @@ int cg_solve(const float *b, float *x, int m, int nx, int ny, int nz, int max_it
     a.ny = ny;
     a.nz = nz;
     a.n = n;
+    a.dnx = make_div32((uint32_t)nx);
+    a.dplane = make_div32((uint32_t)nx * (uint32_t)ny);
     a.G = (int)std::min<uint64_t>((n + kCgThreads - 1) / kCgThreads, kCgBlocks);
     a.max_iter = max_iter;
     a.term = end_res * end_res;
@@ -695,7 +763,7 @@ dim3 stride_grid(uint64_t n) {
 int check_dims(int nx, int ny, int nz, int minimum) {
     GC_ARG(nx >= minimum && ny >= minimum && nz >= minimum, "grid sizes must be >= %d (got %d %d %d)",
            minimum, nx, ny, nz);
-    GC_ARG((uint64_t)nx * ny * nz < (1ull << 40), "grid too large");
+    GC_ARG((uint64_t)nx * ny * nz < (1ull << 32), "grid too large (2^32 points or more)");
     return GPUCAD_OK;
 }
 
