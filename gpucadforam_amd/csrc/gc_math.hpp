@@ -84,4 +84,22 @@ __device__ __forceinline__ float atan2_poly(float y, float x) {
     return t;
 }
 
+// Correctly rounded square root.  v_sqrt_f32 is within 1 ulp for inputs of
+// at least 2^-96; Tuckerman's rounding test (Markstein 1990) then picks the
+// correctly rounded neighbour from the fma residuals of s - ulp and s + ulp:
+//   s - ulp  if x - (s - ulp) s <= 0,   s + ulp  if x - (s + ulp) s > 0.
+// This equals sqrtf on that range (and on +inf, where both residuals are
+// NaN).  A wave with a lane below 2^-96 (zero and NaN included) takes
+// sqrtf itself, which rescales small inputs and patches zero.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    if (__ballot(!(x >= 0x1p-96f)) != 0) return __builtin_sqrtf(x);
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rd = __builtin_fmaf(-sd, s, x);
+    const float ru = __builtin_fmaf(-su, s, x);
+    const float r = rd <= 0.0f ? sd : s;
+    return ru > 0.0f ? su : r;
+}
+
 }  // namespace gc

@@ -353,10 +353,10 @@ __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__res
             float s;
             const float pz = e[ZT_PRIMZ];
             if (prim == GC_PRIM_SPHERE) {
-                s = __builtin_sqrtf(__builtin_fmaf(pz, pz, pxy)) - c.pr;
+                s = sqrt_rn(__builtin_fmaf(pz, pz, pxy)) - c.pr;
             } else {
                 float mz = __builtin_fmaxf(pz, 0.f);
-                s = __builtin_sqrtf(__builtin_fmaf(mz, mz, pxy)) +
+                s = sqrt_rn(__builtin_fmaf(mz, mz, pxy)) +
                     __builtin_fminf(__builtin_fmaxf(pbxy, pz), 0.f);
             }
             const int op = OP == PRIM_ANY ? c.op : OP;
