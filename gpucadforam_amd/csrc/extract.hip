@@ -700,15 +700,18 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
     for_each_block(q, [&](uint32_t tb) {
         const uint32_t gb = tb >> 1, part = tb & 1;  // count block and its half
         const uint32_t si = src.seg_of(gb);
-        if (a.res[si].ovf) return;  // vertex numbers do not fit 32 bits
+        // the block's bookkeeping, loaded with no branch on it so that its
+        // round trip is shared with the sign loads': the grid's overflow
+        // flag, the block's first triangle and its 8 group counts
+        const uint32_t ovf = a.res[si].ovf;
+        const uint64_t tblock = a.bsum[2 * (uint64_t)gb + 1];
+        const uint4 ga = *(const uint4 *)(a.twave + kGroups * (uint64_t)gb);
+        const uint4 gc4 = *(const uint4 *)(a.twave + kGroups * (uint64_t)gb + 4);
         const Seg s = src.get(si);
         const Grid &g = s.g;
         const uint32_t *signs = (const uint32_t *)(a.signs + s.sign_off);
         const uint64_t k = (uint64_t)(gb - s.block_off) * kBlock + part * kTriWords + (tid >> 1);
         const uint32_t *base = a.base + (uint64_t)s.block_off * kBlock * 2;  // by grid half word
-        // first triangle of the wave (loads issued before the sign loads)
-        uint64_t tnext = a.bsum[2 * (uint64_t)gb + 1];
-        for (uint32_t i = 0; i < part * 4 + wave; ++i) tnext += a.twave[kGroups * (uint64_t)gb + i];
         // ---- per half word (owner lane): the sign half of row (y, z) and the
         // crossing halves of the four rows (y..y+1, z..z+1) touching its
         // cells; a cell's cube index and edge vertex numbers follow from these
@@ -743,8 +746,13 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
                 const uint32_t wf = ldh(signs, i0 + 2 * W2 + RW2, has_y2);
                 const uint32_t wg = ldh(signs, i0 + 2 * RW2, has_z2);
                 const uint32_t wh = ldh(signs, i0 + 2 * RW2 + W2, has_z2);
-                const uint32_t b0 = base[i0], b1 = base[i0 + W2], b2 = base[i0 + RW2],
-                               b3 = base[i0 + RW2 + W2];
+                // the first vertex ids are needed only by halves with active
+                // cells, but are loaded (and kept) unconditionally so that
+                // they share the sign loads' round trip
+                vb0 = base[i0];
+                vb1 = base[i0 + W2];
+                vb2 = base[i0 + RW2];
+                vb3 = base[i0 + RW2 + W2];
                 const uint32_t sa = sh1(wa, na), sb = sh1(wb, nb), sc = sh1(wc, nc),
                                sd = sh1(wd, nd);
                 act = ((wa ^ sa) | (wa ^ wb) | (wa ^ sb) | (wa ^ wc) | (wa ^ sc) | (wa ^ wd) |
@@ -762,13 +770,19 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
                     ex3 = c3.ex; ey3 = c3.ey; ez3 = c3.ez;
                     carry = (c0.sx >> 31) | ((c0.sy >> 31) << 1) | ((c1.sx >> 31) << 2) |
                             ((c1.sy >> 31) << 3) | ((c2.sx >> 31) << 4);
-                    vb0 = b0;
-                    vb1 = b1;
-                    vb2 = b2;
-                    vb3 = b3;
                 }
             }
         }
+        // vertex numbers that do not fit 32 bits: the grid's triangles are
+        // not written
+        const uint64_t tcap = ovf ? 0ull : a.tcap;
+        // the wave's first triangle follows the part * 4 + wave groups before it
+        static_assert(kGroups == 8, "two uint4 loads carry a block's group counts");
+        const uint32_t ng = part * 4 + wave;
+        const uint32_t gcount[kGroups] = {ga.x, ga.y, ga.z, ga.w, gc4.x, gc4.y, gc4.z, gc4.w};
+        uint64_t tnext = tblock;
+#pragma unroll
+        for (uint32_t i = 0; i < kGroups; ++i) tnext += i < ng ? gcount[i] : 0u;
         const uint32_t nact = (uint32_t)__popc(act);
         const uint32_t aincl = wave_scan_incl(nact);
         const uint32_t atot = readlane32(aincl, 63);
@@ -860,7 +874,7 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
                     const uint32_t o = own[te], l = o & 255, jt = o >> 8;
                     const uint64_t pkl = pk[l];
                     const uint64_t t = tnext + te;
-                    if (t < a.tcap) {
+                    if (t < tcap) {
                         const uint32_t e0v = (uint32_t)(pkl >> (12 * jt)) & 15u;
                         const uint32_t e1v = (uint32_t)(pkl >> (12 * jt + 4)) & 15u;
                         const uint32_t e2v = (uint32_t)(pkl >> (12 * jt + 8)) & 15u;
