@@ -131,9 +131,11 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) {
 }
 
 // sign word `idx` of a grid (32-bit word index: a grid has fewer than 2^32
-// words) when `ok`, else 0
+// words) when `ok`, else 0.  The load is unconditional (word 0 stands in),
+// so a run of these has no branches and its loads issue back to back.
 __device__ __forceinline__ uint64_t ldw(const uint64_t *s, uint32_t idx, bool ok) {
-    return ok ? s[idx] : 0ull;
+    const uint64_t v = s[ok ? idx : 0u];
+    return ok ? v : 0ull;
 }
 
 struct WordPos {
@@ -306,6 +308,10 @@ __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a, Que
             const uint64_t bn = ldw(signs, kk + W + 1, has_y && hw);
             const uint64_t wc = ldw(signs, kk + RW, has_z);
             const uint64_t cn = ldw(signs, kk + RW + 1, has_z && hw);
+            const uint64_t wd = ldw(signs, kk + RW + W, has_y && has_z);
+            const uint64_t dn = ldw(signs, kk + RW + W + 1, has_y && has_z && hw);
+            // all eight loads issue before any of the arithmetic on them
+            __builtin_amdgcn_sched_barrier(0);
             const uint64_t sa = shift1(wa, an);
             const uint64_t sb = shift1(wb, bn);
             const uint64_t sc = shift1(wc, cn);
@@ -316,8 +322,7 @@ __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a, Que
             const uint64_t ez = has_z ? (wa ^ wc) & vpt : 0ull;
             vc = popc64(ex) + popc64(ey) + popc64(ez);
             if (has_y && has_z) {
-                const uint64_t wd = signs[kk + RW + W];
-                const uint64_t sd = shift1(wd, ldw(signs, kk + RW + W + 1, hw));
+                const uint64_t sd = shift1(wd, dn);
                 const uint64_t act = active_cells(wa, sa, wb, sb, wc, sc, wd, sd, ved);
                 tc = word_triangles(wa, sa, sb, wb, wc, sc, sd, wd, act);
             }
@@ -498,12 +503,14 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a, Queue
             const WordPos p = word_pos(k, g);
             const bool has_y = p.y + 1 < g.ny, has_z = p.z + 1 < g.nz;
             const uint32_t kk = (uint32_t)k, W = (uint32_t)g.W, RW = (uint32_t)g.ny * W;
+            // every load before any use (one round trip)
             const uint64_t w0 = signs[kk], w0n = ldw(signs, kk + 1, p.w + 1 < g.W);
+            const uint64_t wy = ldw(signs, kk + W, has_y), wz = ldw(signs, kk + RW, has_z);
             const uint64_t vpt = mask_below((int64_t)g.nx - 64 * p.w);
             const uint64_t ved = mask_below((int64_t)g.nx - 1 - 64 * p.w);
             ex = (w0 ^ shift1(w0, w0n)) & ved;
-            ey = has_y ? (w0 ^ signs[kk + W]) & vpt : 0ull;
-            ez = has_z ? (w0 ^ signs[kk + RW]) & vpt : 0ull;
+            ey = has_y ? (w0 ^ wy) & vpt : 0ull;
+            ez = has_z ? (w0 ^ wz) & vpt : 0ull;
             s_p0[tid] = ((uint64_t)p.z * g.ny + p.y) * g.nx + 64 * (uint64_t)p.w;
             s_xyz[tid] = (uint32_t)(64 * p.w) | ((uint32_t)p.y << 16);
             s_z[tid] = (uint32_t)p.z;
@@ -623,8 +630,10 @@ __device__ __forceinline__ uint32_t sh1(uint32_t cur, uint32_t next) {
 __device__ __forceinline__ uint32_t below32(int64_t n) {
     return n <= 0 ? 0u : (n >= 32 ? ~0u : (1u << n) - 1u);
 }
+// half `idx` when `ok`, else 0; unconditional load as ldw
 __device__ __forceinline__ uint32_t ldh(const uint32_t *s, uint32_t idx, bool ok) {
-    return ok ? s[idx] : 0u;
+    const uint32_t v = s[ok ? idx : 0u];
+    return ok ? v : 0u;
 }
 __device__ __forceinline__ uint32_t bit32(uint32_t w, uint32_t b) { return (w >> b) & 1u; }
 
