@@ -15,7 +15,16 @@
  *                       manifold/Euler checks, FP64 twin
  *   - lattice field   : FP64 closed form of the formula (independent code path)
  *   - sin/cos/atan    : libm in double precision, error bound 2e-7
- *   - primitives      : FP64 restatement of the same formulas
+ *   - primitives      : float64 geometry of each shape, scipy Euler rotations
+ *                       (tests/test_oracle_primitives.py)
+ *   - band interp     : linear fields, snapping, mirror symmetry
+ *                       (tests/test_oracle_band.py); parity unpinned for the
+ *                       equal-values branch (t = 1 when p0.z == 0) and the
+ *                       uninitialised-t case (defined as 0)
+ *   - SVL             : operator = sum of Dt Dt^T, linear phases, closed-form
+ *                       upsampling (tests/test_oracle_svl.py)
+ *   - unit-cell lattice, cuFFT spectra, CG reduction order: tolerances only
+ *                       (parity unpinned at bit level)
  *   - .obj writer     : std::map restatement of src/File_output.cu:5-81
  *
  * Built with g++ -O3 -fopenmp -ffp-contract=off: every fused multiply-add is
