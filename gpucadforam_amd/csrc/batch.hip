@@ -174,9 +174,9 @@ int enqueue_field_plan(const HostPlan &h, const DevPlan &d, const Shape &sh, int
     return GPUCAD_OK;
 }
 
-BatchExtract make_batch_extract(const HostPlan &h, const DevPlan &d, const Shape &sh, int cus,
-                                const float *fields, const uint64_t *signs, float *vertices,
-                                uint64_t vcap, uint32_t *triangles, uint64_t tcap) {
+BatchExtract make_batch_extract(const HostPlan &h, const DevPlan &d, const float *fields,
+                                const uint64_t *signs, float *vertices, uint64_t vcap,
+                                uint32_t *triangles, uint64_t tcap) {
     BatchExtract b;
     b.fields = fields;
     b.signs = signs;
@@ -189,14 +189,10 @@ BatchExtract make_batch_extract(const HostPlan &h, const DevPlan &d, const Shape
     b.bsum = d.bsum;
     b.ctl = d.ctl;
     b.res = d.res;
-    b.queues = d.queues;
     b.vertices = vertices;
     b.vcap = vertices ? vcap : 0;
     b.triangles = triangles;
     b.tcap = triangles ? tcap : 0;
-    b.wg_count = cus * sh.xc;
-    b.wg_vert = cus * sh.xv;
-    b.wg_tri = cus * sh.xt;
     return b;
 }
 
@@ -215,7 +211,7 @@ int device_cus(int *cus) {
 
 int default_shape(Shape *s) {
     *s = Shape();
-    return extract_batch_occupancy(&s->xc, &s->xv, &s->xt);
+    return GPUCAD_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -236,8 +232,7 @@ void tune_load_locked() {
     if (!f) return;
     char key[256];
     Shape s;
-    while (std::fscanf(f, "%255s %d %d %d %d %d %d", key, &s.bx, &s.lz, &s.wpc, &s.xc, &s.xv,
-                       &s.xt) == 7)
+    while (std::fscanf(f, "%255s %d %d %d", key, &s.bx, &s.lz, &s.wpc) == 4)
         g_tune[key] = s;
     std::fclose(f);
 }
@@ -259,7 +254,7 @@ void tune_store(const std::string &key, const Shape &s) {
     if (!path || !*path) return;
     FILE *f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d %d %d %d %d\n", key.c_str(), s.bx, s.lz, s.wpc, s.xc, s.xv, s.xt);
+    std::fprintf(f, "%s %d %d %d\n", key.c_str(), s.bx, s.lz, s.wpc);
     std::fclose(f);
 }
 
@@ -318,7 +313,7 @@ struct Batch {
             if (st) return st;
         }
         if (phases_extract) {
-            const BatchExtract be = make_batch_extract(plan, d, shape, cus, fields, signs,
+            const BatchExtract be = make_batch_extract(plan, d, fields, signs,
                                                        vertices, vcap, triangles, tcap);
             st = enqueue_extract_batch(be, phases_extract, s);
         }
@@ -385,46 +380,6 @@ struct Batch {
         plan_items(grids.data(), n, shape, plan);
         int st = upload_items();
         if (st) return st;
-        // extraction: workgroups per CU of each persistent stage, up to what fits
-        int oc = 1, ov = 1, ot = 1;
-        st = extract_batch_occupancy(&oc, &ov, &ot);
-        if (st) return st;
-        st = enqueue(cap, 0, true);  // fields and signs for the extraction timings
-        if (st) return st;
-        float bc = 1e30f, bv = 1e30f;
-        for (int m = 1; m <= std::max(oc, ov); m *= 2) {
-            Shape s = best;
-            s.xc = std::min(m, oc);
-            s.xv = std::min(m, ov);
-            s.xt = std::min(m, ot);
-            shape = s;
-            float ms = 0.f;
-            st = time_runs(EXTRACT_COUNT, false, 2, &ms);
-            if (st) return st;
-            if (ms < bc) {
-                bc = ms;
-                best.xc = s.xc;
-            }
-        }
-        shape = best;
-        st = enqueue(cap, EXTRACT_COUNT, false);  // block sums for the emit timings
-        if (st) return st;
-        for (int m = 1; m <= std::max(ov, ot); m *= 2) {
-            Shape s = best;
-            s.xv = std::min(m, ov);
-            s.xt = std::min(m, ot);
-            shape = s;
-            float ms = 0.f;
-            // emit only: zero the vert/tri queues inside time_runs via enqueue
-            st = time_runs(EXTRACT_EMIT, false, 2, &ms);
-            if (st) return st;
-            if (ms < bv) {
-                bv = ms;
-                best.xv = s.xv;
-                best.xt = s.xt;
-            }
-        }
-        shape = best;
         tune_store(key, shape);
         return GPUCAD_OK;
     }
@@ -558,14 +513,6 @@ int batch_create(const gc_lattice_params *grids, int n, const gc_batch_options *
     sh.bx = o->brick_x ? o->brick_x : 64;
     sh.lz = o->brick_lz ? o->brick_lz : 16;
     sh.wpc = o->waves_per_cu ? o->waves_per_cu : 8;
-    {
-        int oc = 1, ov = 1, ot = 1;
-        st = extract_batch_occupancy(&oc, &ov, &ot);
-        if (st) return bail(st);
-        sh.xc = oc;
-        sh.xv = ov;
-        sh.xt = ot;
-    }
     b->shape = sh;
     plan_items(grids, n, b->shape, b->plan);
     st = b->upload_items();

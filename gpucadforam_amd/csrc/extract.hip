@@ -34,11 +34,10 @@
 //           range starting at the wave's scanned base
 //
 // The kernels see grids as segments: a grid's words occupy a contiguous range
-// of global blocks.  The one-grid entry point launches one block per block of
-// the grid; the many-grid form (batched and served) runs persistent
-// workgroups that take global blocks from a device queue (an atomic counter)
-// and find each block's grid in a block -> grid table.  Both run the same
-// block code, so a grid's mesh does not depend on what it is launched with.
+// of global blocks.  Every launch runs one workgroup per block (two per block
+// for the triangle stage); the many-grid form (batched and served) finds each
+// block's grid in a block -> grid table.  Both run the same block code, so a
+// grid's mesh does not depend on what it is launched with.
 #include "extract_types.hpp"
 
 namespace gc {
@@ -197,31 +196,6 @@ struct ManyGrids {
     __device__ __forceinline__ Seg get(uint32_t i) const { return segs[i]; }
 };
 
-// counter == NULL: the workgroup runs block blockIdx.x.  Otherwise the
-// workgroup is persistent and takes blocks from the counter until it passes
-// n; every workgroup leaves after one failed take.
-struct Queue {
-    uint32_t *counter;
-    uint32_t n;
-};
-
-template <class F>
-__device__ __forceinline__ void for_each_block(const Queue &q, F &&f) {
-    if (!q.counter) {
-        f((uint32_t)blockIdx.x);
-        return;
-    }
-    __shared__ uint32_t s_item;
-    for (;;) {
-        __syncthreads();  // the previous block is done with LDS
-        if (threadIdx.x == 0) s_item = atomicAdd(q.counter, 1u);
-        __syncthreads();
-        const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_item);
-        if (b >= q.n) break;
-        f(b);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // signs
 // ---------------------------------------------------------------------------
@@ -288,10 +262,11 @@ struct CountArgs {
 };
 
 template <class Src>
-__global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a, Queue q) {
+__global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a) {
     __shared__ uint32_t s_sum[4][2];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for_each_block(q, [&](uint32_t gb) {
+    {
+        const uint32_t gb = blockIdx.x;
         const Seg s = src.get(src.seg_of(gb));
         const Grid &g = s.g;
         const uint64_t *signs = a.signs + s.sign_off;
@@ -344,7 +319,7 @@ __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a, Que
             a.bsum[2 * (uint64_t)gb + 1] =
                 (uint64_t)s_sum[0][1] + s_sum[1][1] + s_sum[2][1] + s_sum[3][1];
         }
-    });
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -481,14 +456,15 @@ struct VertArgs {
 };
 
 template <class Src>
-__global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a, Queue q) {
+__global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a) {
     __shared__ uint16_t s_list[kVertCap + kBlock];  // + one discard slot per thread
     __shared__ uint64_t s_p0[kBlock];   // linear index of the word's first point
     __shared__ uint32_t s_xyz[kBlock];  // first x | y << 16
     __shared__ uint32_t s_z[kBlock];
     __shared__ uint32_t s_wsum[4];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for_each_block(q, [&](uint32_t gb) {
+    {
+        const uint32_t gb = blockIdx.x;
         const uint32_t si = src.seg_of(gb);
         const Seg s = src.get(si);
         const Grid &g = s.g;
@@ -608,7 +584,7 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a, Queue
             }
             __syncthreads();
         }
-    });
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -696,7 +672,7 @@ struct TriArgs {
 };
 
 template <class Src>
-__global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Queue q) {
+__global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a) {
     constexpr int kWaves = kTriThreads / 64;
     __shared__ uint64_t s_tri[256];
     __shared__ uint16_t s_list[kWaves][kCellCap];
@@ -706,7 +682,8 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
     if (threadIdx.x < 256) s_tri[threadIdx.x] = a.tri_packed[threadIdx.x];
     __syncthreads();
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = tid & 1;
-    for_each_block(q, [&](uint32_t tb) {
+    {
+        const uint32_t tb = blockIdx.x;
         const uint32_t gb = tb >> 1, part = tb & 1;  // count block and its half
         const uint32_t si = src.seg_of(gb);
         // the block's bookkeeping, loaded with no branch on it so that its
@@ -897,7 +874,7 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a, Qu
                 __builtin_amdgcn_wave_barrier();
             }
         }
-    });
+    }
 }
 
 }  // namespace
@@ -948,65 +925,51 @@ int enqueue_extract(const gc_extract_args *a, hipStream_t stream) {
     uint64_t *ctl = (uint64_t *)(ws + L.ctl_off);
     SegResult *res = (SegResult *)(ws + L.res_off);
     const dim3 blocks((unsigned)L.nb);
-    const Queue none = {nullptr, 0};
     hipLaunchKernelGGL(count_kernel<OneGrid>, blocks, dim3(kBlock), 0, stream, one,
-                       CountArgs{signs, twave, bsum}, none);
+                       CountArgs{signs, twave, bsum});
     GC_CHECK_LAUNCH();
     hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, stream, bsum, L.nb, ctl,
                        a->d_counts, (const Seg *)nullptr, 1u, res);
     GC_CHECK_LAUNCH();
     const VertArgs va = {a->field, signs, bsum, res, base, a->vertices,
                          a->vertices ? a->vert_capacity : 0};
-    hipLaunchKernelGGL(vert_kernel<OneGrid>, blocks, dim3(kBlock), 0, stream, one, va, none);
+    hipLaunchKernelGGL(vert_kernel<OneGrid>, blocks, dim3(kBlock), 0, stream, one, va);
     GC_CHECK_LAUNCH();
     const TriArgs ta = {signs, base, twave, bsum, tabs->tri_packed, res, a->triangles,
                         a->triangles ? a->tri_capacity : 0};
     hipLaunchKernelGGL(tri_kernel<OneGrid>, dim3(2 * (unsigned)L.nb), dim3(kTriThreads), 0, stream,
-                       one, ta, none);
+                       one, ta);
     GC_CHECK_LAUNCH();
     return GPUCAD_OK;
 }
 
-// Workgroups of each many-grid stage that fit on one CU at once.
-int extract_batch_occupancy(int *count, int *vert, int *tri) {
-    GC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(count, count_kernel<ManyGrids>, kBlock, 0));
-    GC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(vert, vert_kernel<ManyGrids>, kBlock, 0));
-    GC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(tri, tri_kernel<ManyGrids>, kTriThreads, 0));
-    *count = std::max(1, *count);
-    *vert = std::max(1, *vert);
-    *tri = std::max(1, *tri);
-    return GPUCAD_OK;
-}
-
 int enqueue_extract_batch(const BatchExtract &b, int phases, hipStream_t stream) {
-    GC_ARG(b.nseg > 0 && b.nb > 0 && b.segs && b.blk2seg && b.queues, "empty batch");
+    GC_ARG(b.nseg > 0 && b.nb > 0 && b.segs && b.blk2seg, "empty batch");
+    GC_ARG(b.nb <= 0x7fffffffu / 2, "batch too large");
     const DeviceTables *tabs = nullptr;
     int st = device_tables(&tabs);
     if (st) return st;
     const ManyGrids src = {b.segs, b.blk2seg};
-    const uint32_t wmax = b.nb;
+    // one workgroup per block (two per block for the triangle stage), as for
+    // one grid: the hardware dispatcher hands out blocks with no queue
+    // ticket in front of each block's loads
     if (phases & EXTRACT_COUNT) {
-        const unsigned wg = (unsigned)std::max(1, std::min<int>(b.wg_count, (int)wmax));
-        hipLaunchKernelGGL(count_kernel<ManyGrids>, dim3(wg), dim3(kBlock), 0, stream, src,
-                           CountArgs{b.signs, b.twave, b.bsum},
-                           Queue{b.queues + 0, b.nb});
+        hipLaunchKernelGGL(count_kernel<ManyGrids>, dim3(b.nb), dim3(kBlock), 0, stream, src,
+                           CountArgs{b.signs, b.twave, b.bsum});
         GC_CHECK_LAUNCH();
         hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, stream, b.bsum,
                            (uint64_t)b.nb, b.ctl, (uint64_t *)nullptr, b.segs, b.nseg, b.res);
         GC_CHECK_LAUNCH();
     }
     if (phases & EXTRACT_EMIT) {
-        const unsigned wv = (unsigned)std::max(1, std::min<int>(b.wg_vert, (int)wmax));
         const VertArgs va = {b.fields, b.signs, b.bsum, b.res, b.base, b.vertices,
                              b.vertices ? b.vcap : 0};
-        hipLaunchKernelGGL(vert_kernel<ManyGrids>, dim3(wv), dim3(kBlock), 0, stream, src, va,
-                           Queue{b.queues + 1, b.nb});
+        hipLaunchKernelGGL(vert_kernel<ManyGrids>, dim3(b.nb), dim3(kBlock), 0, stream, src, va);
         GC_CHECK_LAUNCH();
-        const unsigned wt = (unsigned)std::max(1, std::min<int>(b.wg_tri, 2 * (int)wmax));
         const TriArgs ta = {b.signs, b.base, b.twave, b.bsum, tabs->tri_packed, b.res,
                             b.triangles, b.triangles ? b.tcap : 0};
-        hipLaunchKernelGGL(tri_kernel<ManyGrids>, dim3(wt), dim3(kTriThreads), 0, stream, src, ta,
-                           Queue{b.queues + 2, 2 * b.nb});
+        hipLaunchKernelGGL(tri_kernel<ManyGrids>, dim3(2 * b.nb), dim3(kTriThreads), 0, stream,
+                           src, ta);
         GC_CHECK_LAUNCH();
     }
     return GPUCAD_OK;

@@ -66,9 +66,9 @@ struct SegResult {
     uint32_t ovf, pad;
 };
 
-// Extraction of many grids through persistent, queue-fed launches.  The sign
-// words must already hold [f < iso] for every grid (the batched field kernel
-// writes them).
+// Extraction of many grids (one workgroup per global block).  The sign words
+// must already hold [f < iso] for every grid (the batched field kernel writes
+// them).
 struct BatchExtract {
     const float *fields;       // field buffer base
     const uint64_t *signs;     // sign buffer base
@@ -79,12 +79,10 @@ struct BatchExtract {
     uint64_t *bsum;            // nb * 2
     uint64_t *ctl;             // 8 x uint64: totals, any overflow
     SegResult *res;            // device, nseg
-    uint32_t *queues;          // 3 counters (count, vert, tri), zero before the launch
     float *vertices;
     uint64_t vcap;
     uint32_t *triangles;
     uint64_t tcap;
-    int wg_count, wg_vert, wg_tri;  // workgroups of the persistent launches
 };
 
 enum { EXTRACT_COUNT = 1, EXTRACT_EMIT = 2, EXTRACT_ALL = 3 };

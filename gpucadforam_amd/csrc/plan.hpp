@@ -11,14 +11,13 @@
 namespace gc {
 
 constexpr int kVariants = 12;  // field variants: rotation x tpms x thickness modes
-constexpr int kQueues = 16;    // queue counters: 0..2 extraction stages, 3.. field variants
+constexpr int kQueues = 16;    // queue counters: 3.. field variants (0..2 unused)
 
 inline uint64_t round_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
 struct Shape {
     int bx = 64, lz = 16, wpc = 8;   // field brick (x points per wave row, z planes) and
                                      // resident waves per CU of the field launches
-    int xc = 2, xv = 2, xt = 2;      // extraction workgroups per CU: count, vert, tri
 };
 
 struct HostPlan {
@@ -49,12 +48,11 @@ void plan_segments(const gc_lattice_params *p, int n, const uint64_t *field_off,
 void plan_items(const gc_lattice_params *p, int n, const Shape &sh, HostPlan &h);
 int enqueue_field_plan(const HostPlan &h, const DevPlan &d, const Shape &sh, int cus,
                        float *fields, uint64_t *signs, hipStream_t stream);
-BatchExtract make_batch_extract(const HostPlan &h, const DevPlan &d, const Shape &sh, int cus,
-                                const float *fields, const uint64_t *signs, float *vertices,
-                                uint64_t vcap, uint32_t *triangles, uint64_t tcap);
+BatchExtract make_batch_extract(const HostPlan &h, const DevPlan &d, const float *fields,
+                                const uint64_t *signs, float *vertices, uint64_t vcap,
+                                uint32_t *triangles, uint64_t tcap);
 int zero_queues(const DevPlan &d, hipStream_t stream);
 int device_cus(int *cus);
-int extract_batch_occupancy(int *count, int *vert, int *tri);
 // Default shape: brick 64 x 4 x 16, 8 waves per CU, extraction at full occupancy.
 int default_shape(Shape *s);
 

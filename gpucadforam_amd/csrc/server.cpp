@@ -351,7 +351,7 @@ struct Server {
         if ((st = zero_queues(g.d, sl.stream))) return st;
         if ((st = enqueue_field_plan(g.plan, g.d, shape, cus, g.fields, g.signs, sl.stream)))
             return st;
-        const BatchExtract be = make_batch_extract(g.plan, g.d, shape, cus, g.fields, g.signs,
+        const BatchExtract be = make_batch_extract(g.plan, g.d, g.fields, g.signs,
                                                    nullptr, 0, nullptr, 0);
         if ((st = enqueue_extract_batch(be, EXTRACT_COUNT, sl.stream))) return st;
         GC_HIP(hipMemcpyAsync(P + pat, g.d.res, b_res, hipMemcpyDeviceToHost, sl.stream));
@@ -383,7 +383,7 @@ struct Server {
             (uint32_t *)(arena.base() + g.mesh_off + round_up(g.vtot * 12, kGranule));
         int st = zero_queues(g.d, sl.stream);
         if (st) return st;
-        const BatchExtract be = make_batch_extract(g.plan, g.d, shape, cus, g.fields, g.signs,
+        const BatchExtract be = make_batch_extract(g.plan, g.d, g.fields, g.signs,
                                                    verts, g.vtot, tris, g.ttot);
         if ((st = enqueue_extract_batch(be, EXTRACT_EMIT, sl.stream))) return st;
         GC_HIP(hipEventRecord(sl.ev, sl.stream));
