@@ -372,7 +372,7 @@ def bench_batched(runs):
         "plain_gvoxels_per_s": round(pts / (plain_ms * 1e-3) / 1e9, 3),
         "speedup_vs_plain": round(plain_ms / ms, 2),
         "graph": True,
-        "tuning": {k: tuning[k] for k in ("brick_x", "brick_lz", "waves_per_cu")},
+        "tuning": {k: tuning[k] for k in ("brick_x", "brick_lz")},
         "create_s": round(create_s, 3),
     }
 

@@ -1,15 +1,13 @@
 // Batched form: many lattice grids of uneven sizes and types per run.
 //
 // Field: the grids' bricks (BX x 4*(64/BX) columns x lz planes, lattice_body.hpp)
-// are listed per field variant in a device queue; one persistent launch per
-// variant present runs waves_per_cu/4 workgroups per CU that take bricks from
-// an atomic counter until the queue is empty.  Extraction: the grids' word
-// blocks are numbered globally (each grid a contiguous range), and the
-// count, vert and tri stages run as persistent launches over a queue of
-// global blocks (extract.hip); the block scan produces one vertex and one
-// triangle range per grid in the batch's mesh buffers.  A run starts with a
-// small kernel that zeroes the queue counters (no memset node, so the run can
-// be captured once as a HIP graph and replayed).
+// are listed per field variant as (grid, bx, by, bz) codes; one launch per
+// variant present runs one workgroup per brick.  Extraction: the grids' word
+// blocks are numbered globally (each grid a contiguous range), and the count,
+// vert and tri stages run one workgroup per global block (extract.hip); the
+// block scan produces one vertex and one triangle range per grid in the
+// batch's mesh buffers.  A run has no memset node, so it can be captured once
+// as a HIP graph and replayed.
 //
 // Brick shape (BX, lz) and waves per CU of the field launch, and the
 // workgroups per CU of the extraction launches, are picked by timing
@@ -36,31 +34,21 @@ int variant_index(const gc_lattice_params &p) { return field_variant(&p).index()
 template <bool ROT, int TPMS, bool THICK, int BX>
 __global__ void __launch_bounds__(64 * kFieldWaves)
 batch_field_kernel(const FieldConsts *__restrict__ consts, const uint64_t *__restrict__ items,
-                   uint32_t n, uint32_t *counter, float *fields, uint64_t *signs) {
+                   uint32_t n, float *fields, uint64_t *signs) {
     __shared__ float ztab[kMaxLZ * kZTab];
-    __shared__ uint32_t s_item;
-    for (;;) {
-        __syncthreads();  // the previous brick is done with LDS
-        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t it = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_item);
-        if (it >= n) break;
-        const uint64_t code = items[it];
-        const uint32_t seg = (uint32_t)(code & 0xffff);
-        const int bx = (int)((code >> 16) & 0xffff), by = (int)((code >> 32) & 0xffff),
-                  bz = (int)(code >> 48);
-        const FieldConsts &c = consts[seg];
-        lattice_brick<ROT, TPMS, THICK, BX>(c, fields + c.field_off, signs + c.sign_off, bx * BX,
-                                            by * kFieldWaves * (64 / BX), bz * c.lz, ztab);
-    }
+    const uint32_t it = blockIdx.x;  // one workgroup per brick
+    if (it >= n) return;
+    const uint64_t code = items[it];
+    const uint32_t seg = (uint32_t)(code & 0xffff);
+    const int bx = (int)((code >> 16) & 0xffff), by = (int)((code >> 32) & 0xffff),
+              bz = (int)(code >> 48);
+    const FieldConsts &c = consts[seg];
+    lattice_brick<ROT, TPMS, THICK, BX>(c, fields + c.field_off, signs + c.sign_off, bx * BX,
+                                        by * kFieldWaves * (64 / BX), bz * c.lz, ztab);
 }
 
-__global__ void zero_u32_kernel(uint32_t *p, int n) {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
-}
-
-typedef void (*BatchFieldKernel)(const FieldConsts *, const uint64_t *, uint32_t, uint32_t *,
-                                 float *, uint64_t *);
+typedef void (*BatchFieldKernel)(const FieldConsts *, const uint64_t *, uint32_t, float *,
+                                 uint64_t *);
 
 template <bool ROT, int TPMS, bool THICK>
 BatchFieldKernel pick_bx(int bx) {
@@ -159,16 +147,14 @@ void plan_items(const gc_lattice_params *p, int n, const Shape &sh, HostPlan &h)
     h.vbegin[kVariants] = (uint32_t)h.items.size();
 }
 
-// Enqueues the field launches of a plan (queues must be zero).
-int enqueue_field_plan(const HostPlan &h, const DevPlan &d, const Shape &sh, int cus,
-                       float *fields, uint64_t *signs, hipStream_t stream) {
+// Enqueues the field launches of a plan.
+int enqueue_field_plan(const HostPlan &h, const DevPlan &d, const Shape &sh, float *fields,
+                       uint64_t *signs, hipStream_t stream) {
     for (int v = 0; v < kVariants; ++v) {
         const uint32_t n = h.vbegin[v + 1] - h.vbegin[v];
         if (!n) continue;
-        const uint32_t wg = std::max(1u, std::min(n, (uint32_t)(cus * sh.wpc / kFieldWaves)));
-        hipLaunchKernelGGL(field_kernel_for(v, sh.bx), dim3(wg), dim3(64 * kFieldWaves), 0,
-                           stream, d.consts, d.items + h.vbegin[v], n, d.queues + 3 + v, fields,
-                           signs);
+        hipLaunchKernelGGL(field_kernel_for(v, sh.bx), dim3(n), dim3(64 * kFieldWaves), 0,
+                           stream, d.consts, d.items + h.vbegin[v], n, fields, signs);
         GC_CHECK_LAUNCH();
     }
     return GPUCAD_OK;
@@ -194,12 +180,6 @@ BatchExtract make_batch_extract(const HostPlan &h, const DevPlan &d, const float
     b.triangles = triangles;
     b.tcap = triangles ? tcap : 0;
     return b;
-}
-
-int zero_queues(const DevPlan &d, hipStream_t stream) {
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, d.queues, kQueues);
-    GC_CHECK_LAUNCH();
-    return GPUCAD_OK;
 }
 
 int device_cus(int *cus) {
@@ -232,7 +212,7 @@ void tune_load_locked() {
     if (!f) return;
     char key[256];
     Shape s;
-    while (std::fscanf(f, "%255s %d %d %d", key, &s.bx, &s.lz, &s.wpc) == 4)
+    while (std::fscanf(f, "%255s %d %d", key, &s.bx, &s.lz) == 3)
         g_tune[key] = s;
     std::fclose(f);
 }
@@ -254,7 +234,7 @@ void tune_store(const std::string &key, const Shape &s) {
     if (!path || !*path) return;
     FILE *f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d %d\n", key.c_str(), s.bx, s.lz, s.wpc);
+    std::fprintf(f, "%s %d %d\n", key.c_str(), s.bx, s.lz);
     std::fclose(f);
 }
 
@@ -306,10 +286,9 @@ struct Batch {
     hipEvent_t done = nullptr, t0 = nullptr, t1 = nullptr;
 
     int enqueue(hipStream_t s, int phases_extract, bool field = true) {
-        int st = zero_queues(d, s);
-        if (st) return st;
+        int st = GPUCAD_OK;
         if (field) {
-            st = enqueue_field_plan(plan, d, shape, cus, fields, signs, s);
+            st = enqueue_field_plan(plan, d, shape, fields, signs, s);
             if (st) return st;
         }
         if (phases_extract) {
@@ -347,35 +326,30 @@ struct Batch {
     }
 
     int tune(const std::string &key) {
-        const bool fix_bx = opt.brick_x != 0, fix_lz = opt.brick_lz != 0,
-                   fix_w = opt.waves_per_cu != 0;
+        const bool fix_bx = opt.brick_x != 0, fix_lz = opt.brick_lz != 0;
         const std::vector<int> bxs = fix_bx ? std::vector<int>{opt.brick_x}
                                             : std::vector<int>{64, 32, 16, 8};
         const std::vector<int> lzs = fix_lz ? std::vector<int>{opt.brick_lz}
                                             : std::vector<int>{8, 16, 32};
-        const std::vector<int> wpcs = fix_w ? std::vector<int>{opt.waves_per_cu}
-                                            : std::vector<int>{8, 16, 32};
         Shape best = shape;
         float best_ms = 1e30f;
         for (int bx : bxs)
-            for (int lz : lzs)
-                for (int w : wpcs) {
-                    Shape s = shape;
-                    s.bx = bx;
-                    s.lz = lz;
-                    s.wpc = w;
-                    shape = s;
-                    plan_items(grids.data(), n, shape, plan);
-                    int st = upload_items();
-                    if (st) return st;
-                    float ms = 0.f;
-                    st = time_runs(0, true, 2, &ms);
-                    if (st) return st;
-                    if (ms < best_ms) {
-                        best_ms = ms;
-                        best = s;
-                    }
+            for (int lz : lzs) {
+                Shape s = shape;
+                s.bx = bx;
+                s.lz = lz;
+                shape = s;
+                plan_items(grids.data(), n, shape, plan);
+                int st = upload_items();
+                if (st) return st;
+                float ms = 0.f;
+                st = time_runs(0, true, 2, &ms);
+                if (st) return st;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = s;
                 }
+            }
         shape = best;
         plan_items(grids.data(), n, shape, plan);
         int st = upload_items();
@@ -461,7 +435,7 @@ int batch_create(const gc_lattice_params *grids, int n, const gc_batch_options *
                p_b2s = part(nb * 4), p_res = part(n * sizeof(SegResult)),
                p_base = part(nb * kBaseBytesPerBlock),
                p_twave = part(nb * kTwaveBytesPerBlock), p_bsum = part(nb * 16),
-               p_ctl = part(64), p_q = part(kQueues * 4), p_fields = part(fo * 4),
+               p_ctl = part(64), p_fields = part(fo * 4),
                p_signs = part(so * 8);
     hipError_t he = hipMalloc(&b->arena, off);
     if (he != hipSuccess) {
@@ -478,7 +452,6 @@ int batch_create(const gc_lattice_params *grids, int n, const gc_batch_options *
     b->d.twave = (uint32_t *)(A + p_twave.off);
     b->d.bsum = (uint64_t *)(A + p_bsum.off);
     b->d.ctl = (uint64_t *)(A + p_ctl.off);
-    b->d.queues = (uint32_t *)(A + p_q.off);
     b->fields = (float *)(A + p_fields.off);
     b->signs = (uint64_t *)(A + p_signs.off);
     // bricks narrower than 64 leave the tail bytes of a row's last word
@@ -512,7 +485,6 @@ int batch_create(const gc_lattice_params *grids, int n, const gc_batch_options *
     Shape sh;
     sh.bx = o->brick_x ? o->brick_x : 64;
     sh.lz = o->brick_lz ? o->brick_lz : 16;
-    sh.wpc = o->waves_per_cu ? o->waves_per_cu : 8;
     b->shape = sh;
     plan_items(grids, n, b->shape, b->plan);
     st = b->upload_items();
@@ -536,14 +508,13 @@ int batch_create(const gc_lattice_params *grids, int n, const gc_batch_options *
         return bail(fail(GPUCAD_E_OOM, "mesh allocation failed (%llu vertices, %llu triangles)",
                          (unsigned long long)b->vcap, (unsigned long long)b->tcap));
     // ---- shape: given, cached or tuned
-    const bool want_tune = !o->brick_x || !o->brick_lz || !o->waves_per_cu;
+    const bool want_tune = !o->brick_x || !o->brick_lz;
     const std::string key = batch_signature(grids, n, b->cus);
     Shape cached;
     if (want_tune && tune_lookup(key, &cached)) {
         b->shape = cached;
         if (o->brick_x) b->shape.bx = o->brick_x;
         if (o->brick_lz) b->shape.lz = o->brick_lz;
-        if (o->waves_per_cu) b->shape.wpc = o->waves_per_cu;
     } else if (want_tune) {
         st = b->tune(key);
         if (st) return bail(st);
@@ -656,7 +627,7 @@ int gpucad_batch_tuning(const gc_batch *h, gc_batch_options *chosen) {
     *chosen = b->opt;
     chosen->brick_x = b->shape.bx;
     chosen->brick_lz = b->shape.lz;
-    chosen->waves_per_cu = b->shape.wpc;
+    chosen->waves_per_cu = 0;  // not a tuning dimension: one workgroup per brick
     chosen->vert_capacity = b->vcap;
     chosen->tri_capacity = b->tcap;
     return GPUCAD_OK;

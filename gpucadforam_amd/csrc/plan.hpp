@@ -11,13 +11,11 @@
 namespace gc {
 
 constexpr int kVariants = 12;  // field variants: rotation x tpms x thickness modes
-constexpr int kQueues = 16;    // queue counters: 3.. field variants (0..2 unused)
 
 inline uint64_t round_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
 struct Shape {
-    int bx = 64, lz = 16, wpc = 8;   // field brick (x points per wave row, z planes) and
-                                     // resident waves per CU of the field launches
+    int bx = 64, lz = 16;  // field brick: x points per wave row, z planes
 };
 
 struct HostPlan {
@@ -38,7 +36,6 @@ struct DevPlan {
     SegResult *res = nullptr;
     uint32_t *base = nullptr, *twave = nullptr;
     uint64_t *bsum = nullptr, *ctl = nullptr;
-    uint32_t *queues = nullptr;
 };
 
 int validate_batch_grid(const gc_lattice_params &p, int i);
@@ -46,14 +43,13 @@ uint64_t brick_count(const gc_lattice_params &p, int bx, int lz);
 void plan_segments(const gc_lattice_params *p, int n, const uint64_t *field_off,
                    const uint64_t *sign_off, float iso, int unit_domain, HostPlan &h);
 void plan_items(const gc_lattice_params *p, int n, const Shape &sh, HostPlan &h);
-int enqueue_field_plan(const HostPlan &h, const DevPlan &d, const Shape &sh, int cus,
-                       float *fields, uint64_t *signs, hipStream_t stream);
+int enqueue_field_plan(const HostPlan &h, const DevPlan &d, const Shape &sh, float *fields,
+                       uint64_t *signs, hipStream_t stream);
 BatchExtract make_batch_extract(const HostPlan &h, const DevPlan &d, const float *fields,
                                 const uint64_t *signs, float *vertices, uint64_t vcap,
                                 uint32_t *triangles, uint64_t tcap);
-int zero_queues(const DevPlan &d, hipStream_t stream);
 int device_cus(int *cus);
-// Default shape: brick 64 x 4 x 16, 8 waves per CU, extraction at full occupancy.
+// Default shape: brick 64 x 4 x 16.
 int default_shape(Shape *s);
 
 }  // namespace gc

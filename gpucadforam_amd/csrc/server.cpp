@@ -226,7 +226,7 @@ struct Server {
             if (sl.group >= 0) continue;
             Group g;
             g.id = next_group;
-            uint64_t work = 2 * kGranule;  // ctl, queues
+            uint64_t work = 2 * kGranule;  // ctl and one granule of slack
             while (!fifo.empty() && (int)g.tickets.size() < std::min(opt.max_batch, group_cap)) {
                 Request &r = reqs[fifo.front()];
                 const uint64_t fb = field_bytes(r.p), wb = work_bytes(r.p, shape);
@@ -307,7 +307,6 @@ struct Server {
         const uint64_t o_twave = take(nb * kTwaveBytesPerBlock);
         const uint64_t o_bsum = take(nb * 16);
         const uint64_t o_ctl = take(64);
-        const uint64_t o_q = take(kQueues * 4);
         if (at > g.work_off + g.work_size)
             return fail(GPUCAD_E_ARG, "internal: work region overrun");
         g.d.consts = (FieldConsts *)(A + o_consts);
@@ -319,7 +318,6 @@ struct Server {
         g.d.twave = (uint32_t *)(A + o_twave);
         g.d.bsum = (uint64_t *)(A + o_bsum);
         g.d.ctl = (uint64_t *)(A + o_ctl);
-        g.d.queues = (uint32_t *)(A + o_q);
         g.fields = (float *)A;
         g.signs = (uint64_t *)A;
         // tables through pinned staging (the staging also receives the results)
@@ -348,8 +346,7 @@ struct Server {
                                       gpucad_sign_words(p.nx, p.ny, p.nz) * 8, sl.stream));
             }
         }
-        if ((st = zero_queues(g.d, sl.stream))) return st;
-        if ((st = enqueue_field_plan(g.plan, g.d, shape, cus, g.fields, g.signs, sl.stream)))
+        if ((st = enqueue_field_plan(g.plan, g.d, shape, g.fields, g.signs, sl.stream)))
             return st;
         const BatchExtract be = make_batch_extract(g.plan, g.d, g.fields, g.signs,
                                                    nullptr, 0, nullptr, 0);
@@ -381,11 +378,10 @@ struct Server {
         float *verts = (float *)(arena.base() + g.mesh_off);
         uint32_t *tris =
             (uint32_t *)(arena.base() + g.mesh_off + round_up(g.vtot * 12, kGranule));
-        int st = zero_queues(g.d, sl.stream);
-        if (st) return st;
         const BatchExtract be = make_batch_extract(g.plan, g.d, g.fields, g.signs,
                                                    verts, g.vtot, tris, g.ttot);
-        if ((st = enqueue_extract_batch(be, EXTRACT_EMIT, sl.stream))) return st;
+        int st = enqueue_extract_batch(be, EXTRACT_EMIT, sl.stream);
+        if (st) return st;
         GC_HIP(hipEventRecord(sl.ev, sl.stream));
         g.phase = G_EMIT;
         return GPUCAD_OK;
@@ -558,7 +554,6 @@ int gpucad_server_create(const gc_server_options *o, gc_server **out) {
     }
     if (o->brick_x) s->shape.bx = o->brick_x;
     if (o->brick_lz) s->shape.lz = o->brick_lz;
-    if (o->waves_per_cu) s->shape.wpc = o->waves_per_cu;
     s->slots.resize(s->opt.slots);
     for (auto &sl : s->slots) {
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||

@@ -186,8 +186,9 @@ typedef struct gc_batch_options {
     float iso;                 /* isovalue of every grid */
     int32_t brick_x;           /* points along x per wave row: 64, 32, 16, 8; 0 = autotuned */
     int32_t brick_lz;          /* z planes per brick, 1..128; 0 = autotuned */
-    int32_t waves_per_cu;      /* resident waves per CU of the persistent field
-                                  launch, 4..32 in steps of 4; 0 = autotuned */
+    int32_t waves_per_cu;      /* accepted for compatibility (0 or 4..32 in steps
+                                  of 4) and ignored: the field runs one
+                                  workgroup per brick; reported as 0 */
     int32_t use_graph;         /* nonzero: a run is one replay of a captured graph */
     int32_t unit_domain;       /* vertex positions: 0 = point indices, 1 = [0,1]^3
                                   (spacing 1/(n-1) per axis) */
@@ -255,7 +256,7 @@ typedef struct gc_server_options {
     float iso;
     int32_t brick_x;           /* field brick (as gc_batch_options); 0 = 64 */
     int32_t brick_lz;          /* 0 = 16 */
-    int32_t waves_per_cu;      /* 0 = 8 */
+    int32_t waves_per_cu;      /* ignored (see gc_batch_options) */
 } gc_server_options;
 
 typedef struct gc_request_result {
