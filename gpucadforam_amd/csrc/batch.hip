@@ -126,7 +126,7 @@ void plan_segments(const gc_lattice_params *p, int n, const uint64_t *field_off,
     h.nb = block;
 }
 
-// Brick queue of a shape, grouped by field variant.
+// Brick list of a shape, grouped by field variant.
 void plan_items(const gc_lattice_params *p, int n, const Shape &sh, HostPlan &h) {
     h.items.clear();
     for (int i = 0; i < n; ++i) h.consts[i].lz = sh.lz;
@@ -300,7 +300,7 @@ struct Batch {
     }
 
     int upload_items() {
-        if (plan.items.size() > items_cap) return fail(GPUCAD_E_ARG, "brick queue too long");
+        if (plan.items.size() > items_cap) return fail(GPUCAD_E_ARG, "brick list too long");
         GC_HIP(hipMemcpy(d.consts, plan.consts.data(), plan.consts.size() * sizeof(FieldConsts),
                          hipMemcpyHostToDevice));
         GC_HIP(hipMemcpy(items_buf, plan.items.data(), plan.items.size() * sizeof(uint64_t),

@@ -951,8 +951,7 @@ int enqueue_extract_batch(const BatchExtract &b, int phases, hipStream_t stream)
     if (st) return st;
     const ManyGrids src = {b.segs, b.blk2seg};
     // one workgroup per block (two per block for the triangle stage), as for
-    // one grid: the hardware dispatcher hands out blocks with no queue
-    // ticket in front of each block's loads
+    // one grid
     if (phases & EXTRACT_COUNT) {
         hipLaunchKernelGGL(count_kernel<ManyGrids>, dim3(b.nb), dim3(kBlock), 0, stream, src,
                            CountArgs{b.signs, b.twave, b.bsum});

@@ -1,6 +1,6 @@
 // The native spatially varying TPMS lattice field, evaluated one brick at a
-// time (shared by the one-grid launch in field.hip and the persistent batched
-// launch in batch.hip, so both produce the same bits).
+// time (shared by the one-grid launch in field.hip and the batched launch in
+// batch.hip, so both produce the same bits).
 //
 // A brick is what one workgroup of 4 waves evaluates: each wave owns BX x BY
 // (x, y) columns (BX * BY = 64 lanes, the 4 waves stacked along y, so the
