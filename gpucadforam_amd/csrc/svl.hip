@@ -291,7 +291,8 @@ __global__ void matvec_kernel(const float *d, float *q, int nx, int ny, int nz) 
 // Gratings.cu:875-974, scalars kept on the device)
 // ---------------------------------------------------------------------------
 constexpr int kCgThreads = 256;
-constexpr int kCgBlocks = 512;  // blocks per right-hand side (partial sums)
+constexpr int kCgBlocks = 512;  // blocks of the stand-alone dot product
+constexpr int kCgResident = 1280;  // CG workgroups over all right-hand sides (5 per CU)
 constexpr int kMaxRhs = 4;
 
 struct CgState {
@@ -302,11 +303,12 @@ struct CgState {
     int pad[3];
 };
 
-constexpr int kCgUnroll = 4;  // points per thread and step, all their loads in flight
+constexpr int kCgUnroll = 2;  // points per thread and step, all their loads in flight
+constexpr int kCgStep = kCgThreads * kCgUnroll;
 
 struct CgArgs {
-    int nx, ny, nz, G;
-    uint64_t n;
+    int nx, ny, nz, G;             // G: workgroups per right-hand side, a multiple of 8
+    uint64_t n, C;                 // C: points per workgroup range, a multiple of kCgThreads
     Div32 dnx, dplane;
     float *phi, *r, *q, *d0, *d1;  // m vectors of n each (right-hand side s at s*n)
     double *pq, *prr;              // partial sums: pq[m][G], prr[2][m][G]
@@ -335,12 +337,24 @@ __device__ __forceinline__ double sum_partials(const double *p, int G) {
     return block_sum(v);
 }
 
+// Points [lo, hi) of workgroup blockIdx.x.  Workgroups are dispatched to
+// the 8 XCDs round robin, so range (b % 8) * (G / 8) + b / 8 gives each XCD
+// one contiguous run of z planes: the operator's neighbours (two planes up
+// and down) then sit in the L2 of the XCD that reads them.
+__device__ __forceinline__ void cg_range(const CgArgs &a, uint64_t *lo, uint64_t *hi) {
+    const uint32_t b = blockIdx.x;
+    const uint64_t r = (uint64_t)((b & 7u) * ((uint32_t)a.G >> 3) + (b >> 3));
+    *lo = r * a.C < a.n ? r * a.C : a.n;
+    *hi = *lo + a.C < a.n ? *lo + a.C : a.n;
+}
+
 __global__ void cg_init_kernel(const float *b, CgArgs a) {
     const int s = blockIdx.y;
     const uint64_t off = (uint64_t)s * a.n;
+    uint64_t lo, hi;
+    cg_range(a, &lo, &hi);
     double acc = 0.0;
-    for (uint64_t t = (uint64_t)blockIdx.x * kCgThreads + threadIdx.x; t < a.n;
-         t += (uint64_t)a.G * kCgThreads) {
+    for (uint64_t t = lo + threadIdx.x; t < hi; t += kCgThreads) {
         const float v = b[off + t];
         a.phi[off + t] = 0.0f;
         a.r[off + t] = v;
@@ -384,16 +398,17 @@ __global__ void cg_a_kernel(CgArgs a, int it) {
     const float *dprev = first ? r : ((it - 1) & 1 ? a.d1 : a.d0) + m_off;
     const float bt = first ? 0.0f : beta;
     auto get = [=](int64_t i) { return __builtin_fmaf(bt, dprev[i], r[i]); };
-    // points t0 + u * S (u < kCgUnroll) per step: every load of the step is
-    // issued before its stores; the partial sum adds the points in the order
-    // t0, t0 + S, t0 + 2S, ... as a one-point-per-step loop would
-    const uint32_t S = (uint32_t)a.G * kCgThreads;
+    // points t0 + u * kCgThreads (u < kCgUnroll) per step: every load of the
+    // step is issued before its stores; the partial sum adds a thread's points
+    // in increasing order
+    uint64_t lo, hi;
+    cg_range(a, &lo, &hi);
     double acc = 0.0;
-    for (uint64_t t0 = blockIdx.x * kCgThreads + threadIdx.x; t0 < a.n; t0 += kCgUnroll * S) {
+    for (uint64_t t0 = lo + threadIdx.x; t0 < hi; t0 += kCgStep) {
         float dv[kCgUnroll], qv[kCgUnroll];
 #pragma unroll
         for (int u = 0; u < kCgUnroll; ++u) {
-            const uint32_t t = (uint32_t)(t0 + u * S < a.n ? t0 + u * S : t0);
+            const uint32_t t = (uint32_t)(t0 + u * kCgThreads < hi ? t0 + u * kCgThreads : t0);
             int x, y, z;
             point_xyz(t, a.dnx, a.dplane, &x, &y, &z);
             dv[u] = get((int64_t)t);
@@ -401,8 +416,8 @@ __global__ void cg_a_kernel(CgArgs a, int it) {
         }
 #pragma unroll
         for (int u = 0; u < kCgUnroll; ++u) {
-            const uint64_t t = t0 + u * S;
-            if (t < a.n) {
+            const uint64_t t = t0 + u * kCgThreads;
+            if (t < hi) {
                 if (!first) dcur[t] = dv[u];
                 a.q[m_off + t] = qv[u];
                 acc += (double)dv[u] * (double)qv[u];
@@ -425,13 +440,14 @@ __global__ void cg_b_kernel(CgArgs a, int it) {
     const float *d = (it & 1 ? a.d1 : a.d0) + m_off;
     float *phi = a.phi + m_off, *r = a.r + m_off;
     const float *q = a.q + m_off;
-    const uint32_t S = (uint32_t)a.G * kCgThreads;
+    uint64_t lo, hi;
+    cg_range(a, &lo, &hi);
     double acc = 0.0;
-    for (uint64_t t0 = blockIdx.x * kCgThreads + threadIdx.x; t0 < a.n; t0 += kCgUnroll * S) {
+    for (uint64_t t0 = lo + threadIdx.x; t0 < hi; t0 += kCgStep) {
         float dv[kCgUnroll], pv[kCgUnroll], qv[kCgUnroll], rv[kCgUnroll];
 #pragma unroll
         for (int u = 0; u < kCgUnroll; ++u) {
-            const uint32_t t = (uint32_t)(t0 + u * S < a.n ? t0 + u * S : t0);
+            const uint32_t t = (uint32_t)(t0 + u * kCgThreads < hi ? t0 + u * kCgThreads : t0);
             dv[u] = d[t];
             pv[u] = phi[t];
             qv[u] = q[t];
@@ -439,8 +455,8 @@ __global__ void cg_b_kernel(CgArgs a, int it) {
         }
 #pragma unroll
         for (int u = 0; u < kCgUnroll; ++u) {
-            const uint64_t t = t0 + u * S;
-            if (t < a.n) {
+            const uint64_t t = t0 + u * kCgThreads;
+            if (t < hi) {
                 phi[t] = __builtin_fmaf(alpha, dv[u], pv[u]);
                 const float rn = __builtin_fmaf(-alpha, qv[u], rv[u]);
                 r[t] = rn;
@@ -465,7 +481,11 @@ int cg_solve(const float *b, float *x, int m, int nx, int ny, int nz, int max_it
     a.n = n;
     a.dnx = make_div32((uint32_t)nx);
     a.dplane = make_div32((uint32_t)nx * (uint32_t)ny);
-    a.G = (int)std::min<uint64_t>((n + kCgThreads - 1) / kCgThreads, kCgBlocks);
+    // about kCgResident workgroups in all (one wave of them, none waiting),
+    // fewer when ranges of one step per thread cover the grid
+    const uint64_t steps = (n + kCgStep - 1) / kCgStep;
+    a.G = (int)std::min<uint64_t>((steps + 7) & ~uint64_t(7), std::max(8, (kCgResident / m) & ~7));
+    a.C = ((n + a.G - 1) / a.G + kCgThreads - 1) / kCgThreads * kCgThreads;
     a.max_iter = max_iter;
     a.term = end_res * end_res;
     const uint64_t vec = m * n * sizeof(float);
