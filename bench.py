@@ -20,6 +20,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -66,6 +67,8 @@ def parse():
     ap.add_argument("--calib", action="store_true",
                     help="also run a 512 MiB device copy (counter calibration)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched form")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the drop-in meshing of the synthesised lattice")
     ap.add_argument("--no-serve", action="store_true", help="skip the served form")
     ap.add_argument("--no-svl", action="store_true",
                     help="skip the spatially variant lattice synthesis")
@@ -257,6 +260,102 @@ def bench_svl(n=120, period=30.0, lattice_type="b"):
             "loop_cg_iters_max": max(iters),
             "speedup_vs_loop": round(loop_ms / one_ms, 1),
             "fine_gvoxels_per_s": round(fn / (one_ms * 1e-3) / 1e9, 3)}
+
+
+def bench_dropin(n=120, period=30.0, runs=5, cpu=True):
+    """The reference application's meshing of a synthesised sheet lattice
+    (src/main.cu:3949-4010) through the drop-in entry points: the one-call
+    synthesis on the fine (2n)^3 grid (not timed), then normalise_four (band
+    0.20-0.30) and Isosurface::computeIsosurface_lattice with the caller's
+    buffers as the reference allocates them (five uint[numVoxels] arrays,
+    float4 pos/norm[maxVerts], maxVerts = 4 N^3), isoValue 0.25 on the
+    indicator, vol_two zeros, voxel size 0.5 (src/main.cu:2137).  Both calls
+    are synchronous, as the reference makes them; timed on the host clock.
+    Algorithmic bytes: the three fields read once and 96 B per triangle
+    (float4 pos + norm per corner).  CPU beside it: the oracle's
+    restatement of the same two calls on one thread."""
+    dev = "cuda"
+    cell = torch.empty(61 ** 3, dtype=torch.float32, device=dev)
+    gc.check(gc.lib.gpucad_create_lattice(gc.ptr(cell), 61, 61, 61, 61 ** 3, 0))
+    coeffs = gc.unit_spectrum(cell, 61, 2)
+    p = gc.svl_params(n, lattice_type="b", uniform_type=0, const_period=period,
+                      cg_end_res=0.01, cg_max_iter=500)
+    svl, _ = gc.svl_field(p, coeffs)
+    nx, ny, nz = p.fx, p.fy, p.fz
+    npts = nx * ny * nz
+    cells = (nx - 1) * (ny - 1) * (nz - 1)
+    max_verts = max(4 * npts, 300000)
+    ind = torch.empty(npts, dtype=torch.float32, device=dev)
+    nrm = torch.empty(npts, dtype=torch.float32, device=dev)
+    zeros = torch.zeros(npts, dtype=torch.float32, device=dev)
+    u = [torch.empty(cells, dtype=torch.int32, device=dev) for _ in range(5)]
+    pos = torch.empty((max_verts, 4), dtype=torch.float32, device=dev)
+    norm = torch.empty((max_verts, 4), dtype=torch.float32, device=dev)
+    active, total = C.c_uint32(0), C.c_uint32(0)
+    gs = gc.Uint3(nx, ny, nz)
+    sh = gc.Uint3(1, nx - 1, (nx - 1) * (ny - 1))
+    mk = gc.Uint3(nx - 1, ny - 1, nz - 1)
+    vs, ce = gc.Float3(0.5, 0.5, 0.5), gc.Float3(0.0, 0.0, 0.0)
+
+    def normalise():
+        gc.check(gc.lib.gpucad_normalise_four(gc.ptr(svl), gc.ptr(ind), gc.ptr(nrm), npts, nx, ny,
+                                              nz, 0.2, 0.3))
+
+    def isosurface():
+        gc.check(gc.lib.gpucad_compute_isosurface_lattice(
+            gc.ptr(ind), gc.ptr(pos), gc.ptr(norm), 0.25, cells, gc.ptr(u[0]), gc.ptr(u[1]),
+            gc.ptr(u[2]), gc.ptr(u[3]), gs, sh, mk, vs, ce, C.byref(active), C.byref(total),
+            gc.ptr(u[4]), max_verts, gc.ptr(nrm), gc.ptr(zeros), 0.2, 0.3, 0.0, 0.0))
+
+    normalise()
+    isosurface()
+    torch.cuda.synchronize()
+    t_norm, t_iso = [], []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        normalise()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        isosurface()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_norm.append(t1 - t0)
+        t_iso.append(t2 - t1)
+    ms_norm = 1e3 * statistics.median(t_norm)
+    ms_iso = 1e3 * statistics.median(t_iso)
+    tris = total.value // 3
+    nbytes = 3 * 4 * npts + 96 * tris
+    out = {"workload": "drop-in computeIsosurface_lattice of the synthesised b lattice "
+                       "(period %g, fine %d^3), band 0.20-0.30, isoValue 0.25" % (period, nx),
+           "grid": [nx, ny, nz], "runs": runs,
+           "normalise_four_ms": round(ms_norm, 3),
+           "isosurface_ms": round(ms_iso, 3),
+           "ms": round(ms_norm + ms_iso, 3),
+           "gvoxels_per_s": round(npts / ((ms_norm + ms_iso) * 1e-3) / 1e9, 3),
+           "active_voxels": active.value, "total_verts": total.value,
+           "mtri_per_s": round(tris / (ms_iso * 1e-3) / 1e6, 1),
+           "isosurface_GBs": round(nbytes / (ms_iso * 1e-3) / 1e9, 1),
+           "isosurface_frac_hbm": round(nbytes / (ms_iso * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    if cpu:
+        import numpy as np
+        from oracle import oracle as orc
+        v = svl.cpu().numpy()
+        t0 = time.perf_counter()
+        r_ind, r_nrm = orc.normalise_four(v, nx, ny, nz, 0.2, 0.3)
+        t1 = time.perf_counter()
+        ref = orc.compute_isosurface(orc.MODE_BAND2, r_ind, nx, ny, nz, 0.25,
+                                     voxel_size=(0.5, 0.5, 0.5), vol1=r_nrm,
+                                     vol2=np.zeros_like(r_nrm), lo1=0.2, hi1=0.3,
+                                     max_verts=max_verts)
+        t2 = time.perf_counter()
+        out["cpu_oracle"] = {"threads": 1, "normalise_four_ms": round(1e3 * (t1 - t0), 1),
+                             "isosurface_ms": round(1e3 * (t2 - t1), 1),
+                             "same_counts": (ref["active"], ref["total"]) == (active.value,
+                                                                              total.value)}
+        out["speedup_vs_cpu"] = round(1e3 * (t2 - t0) / (ms_norm + ms_iso), 1)
+    del pos, norm, u
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(p, slab, threads, budget_s=12.0):
@@ -519,6 +618,9 @@ def main():
     if not args.no_svl:
         print("bench: spatially variant lattice synthesis", file=sys.stderr, flush=True)
         rec["svl"] = bench_svl()
+    if not args.no_dropin:
+        print("bench: drop-in meshing of the synthesised lattice", file=sys.stderr, flush=True)
+        rec["dropin"] = bench_dropin(cpu=not args.no_cpu)
     if not args.no_batch:
         print("bench: batched form", file=sys.stderr, flush=True)
         rec["batched"] = bench_batched(args.batch_runs)

@@ -408,11 +408,6 @@ int bool_generate(BGenArgs &a, hipStream_t st) {
     return GPUCAD_OK;
 }
 
-int read_u32(const uint32_t *d, uint32_t *h) {
-    GC_HIP(hipMemcpy(h, d, sizeof(uint32_t), hipMemcpyDeviceToHost));
-    return GPUCAD_OK;
-}
-
 }  // namespace
 }  // namespace gc
 
@@ -505,21 +500,21 @@ int gpucad_compute_isosurface(gc_float4 *pos, gc_float4 *norm, float iso, uint32
     int r = bool_classify(d_voxel_verts, d_voxel_occupied, v, grid_size, grid_size_shift,
                           grid_size_mask, n, iso, iso1, iso2, flags, 0);
     if (r) return r;
-    if ((r = dropin_scan_u32(d_voxel_occupied_scan, d_voxel_occupied, n, 0))) return r;
-    uint32_t last = 0, last_scan = 0;
-    if ((r = read_u32(d_voxel_occupied + (n - 1), &last))) return r;
-    if ((r = read_u32(d_voxel_occupied_scan + (n - 1), &last_scan))) return r;
-    *active_voxels = last + last_scan;
+    // the occupancy scan also writes the compacted voxel list (nothing when
+    // no voxel is active, as the reference's early return leaves it)
+    if ((r = dropin_scan_u32(d_voxel_occupied_scan, d_voxel_occupied, n, 0, d_comp_voxel_array)))
+        return r;
+    uint32_t last[2] = {0, 0};
+    if ((r = dropin_read_last_pair(d_voxel_occupied, d_voxel_occupied_scan, n, last, 0)))
+        return r;
+    *active_voxels = last[0] + last[1];
     if (*active_voxels == 0) {  // src/Isosurface.cu:83-87 (nfacets left as it was)
         *total_verts = 0;
         return GPUCAD_OK;
     }
-    if ((r = dropin_compact(d_comp_voxel_array, d_voxel_occupied, d_voxel_occupied_scan, n, 0)))
-        return r;
     if ((r = dropin_scan_u32(d_voxel_verts_scan, d_voxel_verts, n, 0))) return r;
-    if ((r = read_u32(d_voxel_verts + (n - 1), &last))) return r;
-    if ((r = read_u32(d_voxel_verts_scan + (n - 1), &last_scan))) return r;
-    *total_verts = last + last_scan;
+    if ((r = dropin_read_last_pair(d_voxel_verts, d_voxel_verts_scan, n, last, 0))) return r;
+    *total_verts = last[0] + last[1];
     *nfacets = *total_verts / 3;
     // src/Isosurface.cu:120-121 clears max_verts bytes (not elements) of each
     GC_HIP(hipMemset(pos, 0, max_verts));

@@ -114,23 +114,53 @@ __global__ void scan_bsum_kernel(uint32_t *bsum, uint32_t nb) {
     }
 }
 
-__global__ void scan_apply_kernel(uint32_t *out, const uint32_t *in, uint32_t n,
-                                  const uint32_t *bsum) {
-    // each thread owns kScanItems consecutive elements of the tile
-    uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
-    uint32_t vals[kScanItems];
-    uint32_t acc = 0;
-    for (int k = 0; k < kScanItems; ++k) {
-        uint64_t i = base + k;
-        vals[k] = i < n ? in[i] : 0u;
-        acc += vals[k];
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
     }
-    uint32_t total;
-    uint32_t run = block_exclusive_scan(acc, &total) + bsum[blockIdx.x];
+    return v;
+}
+
+// Tile elements are read and written as kScanItems chunks of kScanThreads
+// consecutive elements (every access coalesced); element k * kScanThreads +
+// threadIdx.x is scanned in wave order through shuffles, and the 4 waves x
+// kScanItems chunk totals by the first wave.  With `comp`, also the
+// compaction of the scanned 0/1 occupancy: comp[out[i]] = i where in[i] != 0.
+__global__ void scan_apply_kernel(uint32_t *out, const uint32_t *in, uint32_t n,
+                                  const uint32_t *bsum, uint32_t *comp) {
+    constexpr int kWaves = kScanThreads / 64;
+    __shared__ uint32_t wtot[kScanItems * kWaves];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x;
+    const int wave = threadIdx.x >> 6;
+    uint32_t vals[kScanItems], incl[kScanItems];
+#pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        uint64_t i = base + k;
-        if (i < n) out[i] = run;
-        run += vals[k];
+        const uint64_t i = base + k * kScanThreads;
+        vals[k] = i < n ? in[i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        incl[k] = wave_inclusive_scan(vals[k]);
+        if ((threadIdx.x & 63) == 63) wtot[k * kWaves + wave] = incl[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < kScanItems * kWaves) {
+        const uint32_t v = wtot[threadIdx.x];
+        const uint32_t e = wave_inclusive_scan(v) - v;
+        wtot[threadIdx.x] = e + bsum[blockIdx.x];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + k * kScanThreads;
+        if (i < n) {
+            const uint32_t e = wtot[k * kWaves + wave] + incl[k] - vals[k];
+            out[i] = e;
+            if (comp && vals[k]) comp[e] = (uint32_t)i;
+        }
     }
 }
 
@@ -217,8 +247,10 @@ __device__ __forceinline__ float3 interp4(float iso, float3 p0, float3 p1, float
     return lerp3(p0, p1, t);
 }
 
-__constant__ int c_edge_a[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3};
-__constant__ int c_edge_b[12] = {1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7};
+// edge e joins corners c_edge_a[e] and c_edge_b[e]; the loops over edges
+// are unrolled, so corner arrays stay in registers
+constexpr int c_edge_a[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3};
+constexpr int c_edge_b[12] = {1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7};
 
 enum { GEN_BAND2 = 0, GEN_BAND1 = 1, GEN_STD = 2 };
 
@@ -233,71 +265,98 @@ struct GenArgs {
     const uint8_t *nverts;
 };
 
+// One wave of 64 active voxels (one per lane): each lane interpolates its
+// 12 edge vertices into LDS as the reference's thread does; then the wave
+// writes its voxels' vertices one vertex per lane.  Vertex j of a voxel goes
+// to scan[voxel] + j with its triangle's normal, and a triangle is written
+// only if its first index is below maxVerts - 3, as in the reference; with
+// the scan of the voxels' vertex counts these are one contiguous range
+// (active voxels are compacted in voxel order), so every store of pos and
+// norm is coalesced.
 template <int MODE>
 __global__ void __launch_bounds__(64) generate_kernel(GenArgs a) {
     __shared__ float3 vl[12][64];
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.active) return;
-    const uint32_t voxel = a.comp[i];
-    const uint3 gp = calc_grid_pos(voxel, a.g.shift, a.g.mask);
-    float3 p;
-    p.x = ((float)gp.x - a.g.center.x) * a.g.vs.x;
-    p.y = ((float)gp.y - a.g.center.y) * a.g.vs.y;
-    p.z = ((float)gp.z - a.g.center.z) * a.g.vs.z;
-    const float3 vs = a.g.vs;
-    float3 v[8];
-    v[0] = p;
-    v[1] = make_float3(p.x + vs.x, p.y + 0.f, p.z + 0.f);
-    v[2] = make_float3(p.x + vs.x, p.y + vs.y, p.z + 0.f);
-    v[3] = make_float3(p.x + 0.f, p.y + vs.y, p.z + 0.f);
-    v[4] = make_float3(p.x + 0.f, p.y + 0.f, p.z + vs.z);
-    v[5] = make_float3(p.x + vs.x, p.y + 0.f, p.z + vs.z);
-    v[6] = make_float3(p.x + vs.x, p.y + vs.y, p.z + vs.z);
-    v[7] = make_float3(p.x + 0.f, p.y + vs.y, p.z + vs.z);
-    float fid[8];
-    corners(a.vol, gp, a.g.size, fid);
-    const uint32_t cube = cube_of(fid, a.iso);
-    const uint32_t nv = a.nverts[cube];
-    if (MODE == GEN_STD) {
-        if (nv == 0) return;  // src/MarchingCubes_kernel.cu:3114
-        for (int e = 0; e < 12; ++e)
-            vl[e][threadIdx.x] =
-                interp4(a.iso, v[c_edge_a[e]], v[c_edge_b[e]], fid[c_edge_a[e]], fid[c_edge_b[e]]);
-    } else {
-        float f1[8];
-        corners(a.vol1, gp, a.g.size, f1);
-        if (MODE == GEN_BAND2) {
-            float f2[8];
-            corners(a.vol2, gp, a.g.size, f2);
-            for (int e = 0; e < 12; ++e) {
-                int ca = c_edge_a[e], cb = c_edge_b[e];
-                vl[e][threadIdx.x] = interp3(a.lo1, a.hi1, v[ca], v[cb], f1[ca], f1[cb], f2[ca],
-                                             f2[cb], a.lo2, a.hi2, fid[ca], fid[cb]);
-            }
+    __shared__ uint8_t owner[64 * 15];  // output vertex -> lane
+    __shared__ uint32_t s_first[64], s_base[64];
+    __shared__ uint8_t s_cube[64];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t i = blockIdx.x * 64u + lane;
+    uint32_t cube = 0, nv = 0, vbase = 0;
+    if (i < a.active) {
+        const uint32_t voxel = a.comp[i];
+        vbase = a.scan[voxel];
+        const uint3 gp = calc_grid_pos(voxel, a.g.shift, a.g.mask);
+        float3 p;
+        p.x = ((float)gp.x - a.g.center.x) * a.g.vs.x;
+        p.y = ((float)gp.y - a.g.center.y) * a.g.vs.y;
+        p.z = ((float)gp.z - a.g.center.z) * a.g.vs.z;
+        const float3 vs = a.g.vs;
+        float3 v[8];
+        v[0] = p;
+        v[1] = make_float3(p.x + vs.x, p.y + 0.f, p.z + 0.f);
+        v[2] = make_float3(p.x + vs.x, p.y + vs.y, p.z + 0.f);
+        v[3] = make_float3(p.x + 0.f, p.y + vs.y, p.z + 0.f);
+        v[4] = make_float3(p.x + 0.f, p.y + 0.f, p.z + vs.z);
+        v[5] = make_float3(p.x + vs.x, p.y + 0.f, p.z + vs.z);
+        v[6] = make_float3(p.x + vs.x, p.y + vs.y, p.z + vs.z);
+        v[7] = make_float3(p.x + 0.f, p.y + vs.y, p.z + vs.z);
+        float fid[8];
+        corners(a.vol, gp, a.g.size, fid);
+        cube = cube_of(fid, a.iso);
+        nv = a.nverts[cube];
+        if (MODE == GEN_STD) {
+            // an active voxel has nv > 0 (src/MarchingCubes_kernel.cu:3114)
+#pragma unroll
+            for (int e = 0; e < 12; ++e)
+                vl[e][lane] = interp4(a.iso, v[c_edge_a[e]], v[c_edge_b[e]], fid[c_edge_a[e]],
+                                      fid[c_edge_b[e]]);
         } else {
-            for (int e = 0; e < 12; ++e) {
-                int ca = c_edge_a[e], cb = c_edge_b[e];
-                vl[e][threadIdx.x] =
-                    interp2(a.lo1, a.hi1, v[ca], v[cb], f1[ca], f1[cb], fid[ca], fid[cb]);
+            float f1[8];
+            corners(a.vol1, gp, a.g.size, f1);
+            if (MODE == GEN_BAND2) {
+                float f2[8];
+                corners(a.vol2, gp, a.g.size, f2);
+#pragma unroll
+                for (int e = 0; e < 12; ++e) {
+                    int ca = c_edge_a[e], cb = c_edge_b[e];
+                    vl[e][lane] = interp3(a.lo1, a.hi1, v[ca], v[cb], f1[ca], f1[cb], f2[ca],
+                                          f2[cb], a.lo2, a.hi2, fid[ca], fid[cb]);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 12; ++e) {
+                    int ca = c_edge_a[e], cb = c_edge_b[e];
+                    vl[e][lane] =
+                        interp2(a.lo1, a.hi1, v[ca], v[cb], f1[ca], f1[cb], fid[ca], fid[cb]);
+                }
             }
         }
     }
-    const uint32_t base = a.scan[voxel];
-    for (uint32_t j = 0; j < nv; j += 3) {
-        const uint32_t index = base + j;
-        const float3 a0 = vl[a.tri[cube * 16 + j]][threadIdx.x];
-        const float3 a1 = vl[a.tri[cube * 16 + j + 1]][threadIdx.x];
-        const float3 a2 = vl[a.tri[cube * 16 + j + 2]][threadIdx.x];
+    const uint32_t incl = wave_inclusive_scan(nv);
+    const uint32_t first = incl - nv;
+    const uint32_t total = __shfl(incl, 63, 64);
+    s_first[lane] = first;
+    s_base[lane] = vbase;
+    s_cube[lane] = (uint8_t)cube;
+    for (uint32_t j = 0; j < nv; ++j) owner[first + j] = (uint8_t)lane;
+    __syncthreads();
+    for (uint32_t k = lane; k < total; k += 64) {
+        const uint32_t o = owner[k];
+        const uint32_t j = k - s_first[o];
+        const uint32_t t0 = j - j % 3u;  // the triangle's first vertex
+        const uint32_t c = s_cube[o];
+        const float3 a0 = vl[a.tri[c * 16 + t0]][o];
+        const float3 a1 = vl[a.tri[c * 16 + t0 + 1]][o];
+        const float3 a2 = vl[a.tri[c * 16 + t0 + 2]][o];
         const float3 e0 = make_float3(a1.x - a0.x, a1.y - a0.y, a1.z - a0.z);
         const float3 e1 = make_float3(a2.x - a0.x, a2.y - a0.y, a2.z - a0.z);
         const float3 n = cross3(e0, e1);
-        if (index < a.max_verts - 3u) {  // uint arithmetic as in the reference
-            a.pos[index] = make_float4(a0.x, a0.y, a0.z, 1.0f);
-            a.norm[index] = make_float4(n.x, n.y, n.z, 0.0f);
-            a.pos[index + 1] = make_float4(a1.x, a1.y, a1.z, 1.0f);
-            a.norm[index + 1] = make_float4(n.x, n.y, n.z, 0.0f);
-            a.pos[index + 2] = make_float4(a2.x, a2.y, a2.z, 1.0f);
-            a.norm[index + 2] = make_float4(n.x, n.y, n.z, 0.0f);
+        const uint32_t index = s_base[o] + t0;  // scan[voxel] + t0, uint arithmetic
+        if (index < a.max_verts - 3u) {
+            const uint32_t r = j - t0;
+            const float3 q = r == 0 ? a0 : (r == 1 ? a1 : a2);
+            a.pos[index + r] = make_float4(q.x, q.y, q.z, 1.0f);
+            a.norm[index + r] = make_float4(n.x, n.y, n.z, 0.0f);
         }
     }
 }
@@ -342,7 +401,8 @@ int classify(uint32_t *verts, uint32_t *occ, const float *vol, gc_uint3 s, gc_ui
     return GPUCAD_OK;
 }
 
-int scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st) {
+int scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st,
+             uint32_t *comp = nullptr) {
     GC_ARG(out && in, "NULL buffer");
     if (n == 0) return GPUCAD_OK;
     uint32_t nb = (uint32_t)(((uint64_t)n + kScanTile - 1) / kScanTile);
@@ -354,7 +414,8 @@ int scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st) {
     GC_CHECK_LAUNCH();
     hipLaunchKernelGGL(scan_bsum_kernel, dim3(1), dim3(kScanThreads), 0, st, bsum, nb);
     GC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(kScanThreads), 0, st, out, in, n, bsum);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(kScanThreads), 0, st, out, in, n, bsum,
+                       comp);
     GC_CHECK_LAUNCH();
     return GPUCAD_OK;
 }
@@ -385,8 +446,25 @@ int generate(GenArgs &a, hipStream_t st) {
     return GPUCAD_OK;
 }
 
-int read_u32(const uint32_t *d, uint32_t *h) {
-    GC_HIP(hipMemcpy(h, d, sizeof(uint32_t), hipMemcpyDeviceToHost));
+__global__ void last_pair_kernel(const uint32_t *a, const uint32_t *b, uint32_t n,
+                                 uint32_t *out) {
+    if (threadIdx.x == 0) {
+        out[0] = a[n - 1];
+        out[1] = b[n - 1];
+    }
+}
+
+// h[0] = a[n-1], h[1] = b[n-1] in one device-to-host copy (the reference
+// copies each with its own cudaMemcpy, Isosurface.cu:424-436, 453-472)
+int read_last_pair(const uint32_t *a, const uint32_t *b, uint32_t n, uint32_t *h,
+                   hipStream_t st) {
+    void *ws = nullptr;
+    int r = scratch(2 * sizeof(uint32_t), &ws);
+    if (r) return r;
+    hipLaunchKernelGGL(last_pair_kernel, dim3(1), dim3(64), 0, st, a, b, n, (uint32_t *)ws);
+    GC_CHECK_LAUNCH();
+    GC_HIP(hipMemcpyAsync(h, ws, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    GC_HIP(hipStreamSynchronize(st));
     return GPUCAD_OK;
 }
 
@@ -403,23 +481,21 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
     hipStream_t st = 0;
     int r = classify(d_verts, d_occ, vol, s, sh, m, n, iso, st);
     if (r) return r;
-    r = scan_u32(d_occ_scan, d_occ, n, st);
+    // the occupancy scan also writes the compacted voxel list (nothing when
+    // no voxel is active, as the reference's early return leaves it)
+    r = scan_u32(d_occ_scan, d_occ, n, st, d_comp);
     if (r) return r;
-    uint32_t last = 0, last_scan = 0;
-    if ((r = read_u32(d_occ + (n - 1), &last))) return r;
-    if ((r = read_u32(d_occ_scan + (n - 1), &last_scan))) return r;
-    *active = last + last_scan;
+    uint32_t last[2] = {0, 0};
+    if ((r = read_last_pair(d_occ, d_occ_scan, n, last, st))) return r;
+    *active = last[0] + last[1];
     if (*active == 0) {
         *total = 0;
         return GPUCAD_OK;
     }
-    r = compact(d_comp, d_occ, d_occ_scan, n, st);
-    if (r) return r;
     r = scan_u32(d_verts_scan, d_verts, n, st);
     if (r) return r;
-    if ((r = read_u32(d_verts + (n - 1), &last))) return r;
-    if ((r = read_u32(d_verts_scan + (n - 1), &last_scan))) return r;
-    *total = last + last_scan;
+    if ((r = read_last_pair(d_verts, d_verts_scan, n, last, st))) return r;
+    *total = last[0] + last[1];
     GenArgs a;
     a.pos = (float4 *)pos;
     a.norm = (float4 *)norm;
@@ -447,12 +523,13 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
 int check_dropin_grid(gc_uint3 s, gc_uint3 sh, gc_uint3 m, uint32_t n) {
     return check_grid(s, sh, m, n);
 }
-int dropin_scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st) {
-    return scan_u32(out, in, n, st);
+int dropin_scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st,
+                    uint32_t *comp) {
+    return scan_u32(out, in, n, st, comp);
 }
-int dropin_compact(uint32_t *comp, const uint32_t *occ, const uint32_t *occ_scan, uint32_t n,
-                   hipStream_t st) {
-    return compact(comp, occ, occ_scan, n, st);
+int dropin_read_last_pair(const uint32_t *a, const uint32_t *b, uint32_t n, uint32_t *h,
+                          hipStream_t st) {
+    return read_last_pair(a, b, n, h, st);
 }
 
 }  // namespace gc

@@ -54,11 +54,16 @@ const uint64_t *host_tri_packed();   // 256
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 // Shared pieces of the drop-in marching cubes (dropin_mc.hip): the grid
-// layout check, the exclusive uint32 scan and the compaction, enqueued on st.
+// layout check, the exclusive uint32 scan (enqueued on st) and the read of
+// the last elements.
 int check_dropin_grid(gc_uint3 s, gc_uint3 sh, gc_uint3 m, uint32_t n);
-int dropin_scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st);
-int dropin_compact(uint32_t *comp, const uint32_t *occ, const uint32_t *occ_scan, uint32_t n,
-                   hipStream_t st);
+// With `comp`, the scan of a 0/1 occupancy also writes the compaction
+// comp[out[i]] = i where in[i] != 0.
+int dropin_scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st,
+                    uint32_t *comp = nullptr);
+// h[0] = a[n-1], h[1] = b[n-1] in one device-to-host copy, after the work on st
+int dropin_read_last_pair(const uint32_t *a, const uint32_t *b, uint32_t n, uint32_t *h,
+                          hipStream_t st);
 
 // Scratch memory owned by the library for drop-in calls (grown on demand,
 // one arena per device, never shrunk).
