@@ -29,6 +29,7 @@
 // the reference); every cell is classified (no 2-D grid fold); a volume the
 // mode does not read may be NULL.
 #include "gc_common.hpp"
+#include "wave_ops.hpp"
 
 namespace gc {
 namespace {
@@ -262,12 +263,13 @@ __device__ float3 interp_new(float iso, float lo, float hi, float3 p0, float3 p1
 
 // edges as the primitive modes read them: from corner a to corner b with the
 // edge parameter of corner c along axis ax (edge 2: corner 3 -> 2, edge 6: 7 -> 6)
-__constant__ int c_pa[12] = {0, 1, 3, 0, 4, 5, 7, 4, 0, 1, 2, 3};
-__constant__ int c_pb[12] = {1, 2, 2, 3, 5, 6, 6, 7, 4, 5, 6, 7};
-__constant__ int c_pax[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+// (the loops over edges are unrolled, so corner arrays stay in registers)
+constexpr int c_pa[12] = {0, 1, 3, 0, 4, 5, 7, 4, 0, 1, 2, 3};
+constexpr int c_pb[12] = {1, 2, 2, 3, 5, 6, 6, 7, 4, 5, 6, 7};
+constexpr int c_pax[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
 // edges of vertexInterp_new (fixed mode): corner pairs in the cube's edge order
-__constant__ int c_na[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3};
-__constant__ int c_nb[12] = {1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7};
+constexpr int c_na[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3};
+constexpr int c_nb[12] = {1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7};
 
 struct BGenArgs {
     float4 *pos, *norm;
@@ -284,76 +286,102 @@ __device__ __forceinline__ float tparam(const GridPoint &p, int ax) {
     return ax == 0 ? p.t_x : (ax == 1 ? p.t_y : p.t_z);
 }
 
+// One lane per active voxel computes its 12 edge vertices into LDS as the
+// reference's thread does; the wave then writes its voxels' vertices one
+// output vertex per lane (contiguous with the scan of the voxels' counts, so
+// the stores are coalesced), vertex j of a voxel at scan[voxel] + j, each
+// triangle only if its first index is below maxVerts - 3.
 __global__ void __launch_bounds__(64) bool_generate_kernel(BGenArgs a) {
     __shared__ float3 vl[12][64];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.active) return;
-    const uint32_t voxel = a.comp[i];
-    const uint3 gp = calc_grid_pos(voxel, a.g.shift, a.g.mask);
-    float3 p;
-    p.x = ((float)gp.x - a.g.center.x) * a.g.vs.x;
-    p.y = ((float)gp.y - a.g.center.y) * a.g.vs.y;
-    p.z = ((float)gp.z - a.g.center.z) * a.g.vs.z;
-    const float3 vs = a.g.vs;
-    float3 v[8];
-    v[0] = p;
-    v[1] = make_float3(p.x + vs.x, p.y + 0.f, p.z + 0.f);
-    v[2] = make_float3(p.x + vs.x, p.y + vs.y, p.z + 0.f);
-    v[3] = make_float3(p.x + 0.f, p.y + vs.y, p.z + 0.f);
-    v[4] = make_float3(p.x + 0.f, p.y + 0.f, p.z + vs.z);
-    v[5] = make_float3(p.x + vs.x, p.y + 0.f, p.z + vs.z);
-    v[6] = make_float3(p.x + vs.x, p.y + vs.y, p.z + vs.z);
-    v[7] = make_float3(p.x + 0.f, p.y + vs.y, p.z + vs.z);
-    bool nf, nd, nl;
-    needs(a.flags, &nf, &nd, &nl);
-    uint32_t idx[8];
-    corner_index(gp, a.g.size, idx);
-    const uint32_t cube = cube_index(a.flags, a.v, idx, a.iso, a.lo, a.hi, nf, nd, nl);
-    const uint32_t nv = a.nverts[cube];
-    if (nv == 0) return;
-    const uint32_t f = a.flags;
-    if (f & GC_BOOL_MAKE_REGION) {
-        for (int e = 0; e < 12; ++e)
-            vl[e][threadIdx.x] =
-                lerp3(v[c_pa[e]], v[c_pb[e]], tparam(a.v.fixed[idx[c_pa[e]]], c_pax[e]));
-    } else if (f & GC_BOOL_FIXED) {
-        for (int e = 0; e < 12; ++e) {
-            const int ca = c_na[e], cb = c_nb[e];
-            vl[e][threadIdx.x] = interp_new(a.iso, a.lo, a.hi, v[ca], v[cb], a.v.dyn[idx[ca]],
-                                            a.v.dyn[idx[cb]], a.v.lat[idx[ca]], a.v.lat[idx[cb]]);
-        }
-    } else if (f & GC_BOOL_DYNAMIC) {
-        for (int e = 0; e < 12; ++e) {
-            const int ca = c_pa[e], cb = c_pb[e];
-            vl[e][threadIdx.x] =
-                interp_primitive_one(a.lo, a.hi, v[ca], v[cb], a.v.lat[idx[ca]], a.v.lat[idx[cb]],
+    __shared__ uint8_t owner[64 * 15];  // output vertex -> lane
+    __shared__ uint32_t s_first[64], s_base[64];
+    __shared__ uint8_t s_cube[64];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t i = blockIdx.x * 64u + lane;
+    uint32_t cube = 0, nv = 0, vbase = 0;
+    if (i < a.active) {
+        const uint32_t voxel = a.comp[i];
+        vbase = a.scan[voxel];
+        const uint3 gp = calc_grid_pos(voxel, a.g.shift, a.g.mask);
+        float3 p;
+        p.x = ((float)gp.x - a.g.center.x) * a.g.vs.x;
+        p.y = ((float)gp.y - a.g.center.y) * a.g.vs.y;
+        p.z = ((float)gp.z - a.g.center.z) * a.g.vs.z;
+        const float3 vs = a.g.vs;
+        float3 v[8];
+        v[0] = p;
+        v[1] = make_float3(p.x + vs.x, p.y + 0.f, p.z + 0.f);
+        v[2] = make_float3(p.x + vs.x, p.y + vs.y, p.z + 0.f);
+        v[3] = make_float3(p.x + 0.f, p.y + vs.y, p.z + 0.f);
+        v[4] = make_float3(p.x + 0.f, p.y + 0.f, p.z + vs.z);
+        v[5] = make_float3(p.x + vs.x, p.y + 0.f, p.z + vs.z);
+        v[6] = make_float3(p.x + vs.x, p.y + vs.y, p.z + vs.z);
+        v[7] = make_float3(p.x + 0.f, p.y + vs.y, p.z + vs.z);
+        bool nf, nd, nl;
+        needs(a.flags, &nf, &nd, &nl);
+        uint32_t idx[8];
+        corner_index(gp, a.g.size, idx);
+        cube = cube_index(a.flags, a.v, idx, a.iso, a.lo, a.hi, nf, nd, nl);
+        nv = a.nverts[cube];
+        const uint32_t f = a.flags;
+        if (nv == 0) {
+            // nothing to write (src/MarchingCubes_kernel.cu:2078)
+        } else if (f & GC_BOOL_MAKE_REGION) {
+#pragma unroll
+            for (int e = 0; e < 12; ++e)
+                vl[e][lane] =
+                    lerp3(v[c_pa[e]], v[c_pb[e]], tparam(a.v.fixed[idx[c_pa[e]]], c_pax[e]));
+        } else if (f & GC_BOOL_FIXED) {
+#pragma unroll
+            for (int e = 0; e < 12; ++e) {
+                const int ca = c_na[e], cb = c_nb[e];
+                vl[e][lane] = interp_new(a.iso, a.lo, a.hi, v[ca], v[cb], a.v.dyn[idx[ca]],
+                                         a.v.dyn[idx[cb]], a.v.lat[idx[ca]], a.v.lat[idx[cb]]);
+            }
+        } else if (f & GC_BOOL_DYNAMIC) {
+#pragma unroll
+            for (int e = 0; e < 12; ++e) {
+                const int ca = c_pa[e], cb = c_pb[e];
+                vl[e][lane] =
+                    interp_primitive_one(a.lo, a.hi, v[ca], v[cb], a.v.lat[idx[ca]],
+                                         a.v.lat[idx[cb]], tparam(a.v.fixed[idx[ca]], c_pax[e]));
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 12; ++e) {
+                const int ca = c_pa[e], cb = c_pb[e];
+                vl[e][lane] =
+                    interp_primitive(a.iso, v[ca], v[cb], a.v.dyn[idx[ca]], a.v.dyn[idx[cb]],
                                      tparam(a.v.fixed[idx[ca]], c_pax[e]));
-        }
-    } else {
-        for (int e = 0; e < 12; ++e) {
-            const int ca = c_pa[e], cb = c_pb[e];
-            vl[e][threadIdx.x] =
-                interp_primitive(a.iso, v[ca], v[cb], a.v.dyn[idx[ca]], a.v.dyn[idx[cb]],
-                                 tparam(a.v.fixed[idx[ca]], c_pax[e]));
+            }
         }
     }
-    const uint32_t base = a.scan[voxel];
-    for (uint32_t j = 0; j < nv; j += 3) {
-        const uint32_t index = base + j;
-        const float3 a0 = vl[a.tri[cube * 16 + j]][threadIdx.x];
-        const float3 a1 = vl[a.tri[cube * 16 + j + 1]][threadIdx.x];
-        const float3 a2 = vl[a.tri[cube * 16 + j + 2]][threadIdx.x];
+    const uint32_t incl = wave_inclusive_scan(nv);
+    const uint32_t first = incl - nv;
+    const uint32_t total = __shfl(incl, 63, 64);
+    s_first[lane] = first;
+    s_base[lane] = vbase;
+    s_cube[lane] = (uint8_t)cube;
+    for (uint32_t j = 0; j < nv; ++j) owner[first + j] = (uint8_t)lane;
+    __syncthreads();
+    for (uint32_t k = lane; k < total; k += 64) {
+        const uint32_t o = owner[k];
+        const uint32_t j = k - s_first[o];
+        const uint32_t t0 = j - j % 3u;  // the triangle's first vertex
+        const uint32_t c = s_cube[o];
+        const float3 a0 = vl[a.tri[c * 16 + t0]][o];
+        const float3 a1 = vl[a.tri[c * 16 + t0 + 1]][o];
+        const float3 a2 = vl[a.tri[c * 16 + t0 + 2]][o];
         // calcNormal(ver[0], ver[2], ver[1]) = cross(v2 - v0, v1 - v0)
         const float3 e0 = make_float3(a2.x - a0.x, a2.y - a0.y, a2.z - a0.z);
         const float3 e1 = make_float3(a1.x - a0.x, a1.y - a0.y, a1.z - a0.z);
         const float3 n = cross3(e0, e1);
-        if (index < a.max_verts - 3u) {  // uint arithmetic as in the reference
-            a.pos[index] = make_float4(a0.x, a0.y, a0.z, 1.0f);
-            a.norm[index] = make_float4(n.x, n.y, n.z, 0.5f);
-            a.pos[index + 1] = make_float4(a1.x, a1.y, a1.z, 1.0f);
-            a.norm[index + 1] = make_float4(n.x, n.y, n.z, 0.5f);
-            a.pos[index + 2] = make_float4(a2.x, a2.y, a2.z, 1.0f);
-            a.norm[index + 2] = make_float4(n.x, n.y, n.z, 0.5f);
+        const uint32_t index = s_base[o] + t0;  // scan[voxel] + t0, uint arithmetic
+        if (index < a.max_verts - 3u) {
+            const uint32_t r = j - t0;
+            const float3 q = r == 0 ? a0 : (r == 1 ? a1 : a2);
+            a.pos[index + r] = make_float4(q.x, q.y, q.z, 1.0f);
+            a.norm[index + r] = make_float4(n.x, n.y, n.z, 0.5f);
         }
     }
 }

@@ -11,6 +11,7 @@
 //  * lerp(a, b, t) is evaluated as fma(t, b - a, a) and cross products as
 //    fma(a.y, b.z, -(a.z*b.y)), ... (fused forms chosen for this build).
 #include "gc_common.hpp"
+#include "wave_ops.hpp"
 
 namespace gc {
 namespace {
@@ -112,16 +113,6 @@ __global__ void scan_bsum_kernel(uint32_t *bsum, uint32_t nb) {
         bsum[i] = run;
         run += v;
     }
-}
-
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(v, o, 64);
-        if (lane >= o) v += u;
-    }
-    return v;
 }
 
 // Tile elements are read and written as kScanItems chunks of kScanThreads
@@ -305,7 +296,7 @@ __global__ void __launch_bounds__(64) generate_kernel(GenArgs a) {
         cube = cube_of(fid, a.iso);
         nv = a.nverts[cube];
         if (MODE == GEN_STD) {
-            // an active voxel has nv > 0 (src/MarchingCubes_kernel.cu:3114)
+            // an active voxel has nv > 0 (src/MarchingCubes_kernel.cu:3103)
 #pragma unroll
             for (int e = 0; e < 12; ++e)
                 vl[e][lane] = interp4(a.iso, v[c_edge_a[e]], v[c_edge_b[e]], fid[c_edge_a[e]],
