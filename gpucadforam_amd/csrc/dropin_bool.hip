@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(64) bool_generate_kernel(BGenArgs a) {
     __shared__ float3 vl[12][64];
     __shared__ uint8_t owner[64 * 15];  // output vertex -> lane
     __shared__ uint32_t s_first[64], s_base[64];
-    __shared__ uint8_t s_cube[64];
+    __shared__ uint4 s_tri[64];  // each lane's triTable row
     const uint32_t lane = threadIdx.x;
     const uint32_t i = blockIdx.x * 64u + lane;
     uint32_t cube = 0, nv = 0, vbase = 0;
@@ -361,17 +361,17 @@ __global__ void __launch_bounds__(64) bool_generate_kernel(BGenArgs a) {
     const uint32_t total = __shfl(incl, 63, 64);
     s_first[lane] = first;
     s_base[lane] = vbase;
-    s_cube[lane] = (uint8_t)cube;
+    s_tri[lane] = *(const uint4 *)(a.tri + cube * 16);
     for (uint32_t j = 0; j < nv; ++j) owner[first + j] = (uint8_t)lane;
     __syncthreads();
     for (uint32_t k = lane; k < total; k += 64) {
         const uint32_t o = owner[k];
         const uint32_t j = k - s_first[o];
         const uint32_t t0 = j - j % 3u;  // the triangle's first vertex
-        const uint32_t c = s_cube[o];
-        const float3 a0 = vl[a.tri[c * 16 + t0]][o];
-        const float3 a1 = vl[a.tri[c * 16 + t0 + 1]][o];
-        const float3 a2 = vl[a.tri[c * 16 + t0 + 2]][o];
+        const uint8_t *row = (const uint8_t *)&s_tri[o];
+        const float3 a0 = vl[row[t0]][o];
+        const float3 a1 = vl[row[t0 + 1]][o];
+        const float3 a2 = vl[row[t0 + 2]][o];
         // calcNormal(ver[0], ver[2], ver[1]) = cross(v2 - v0, v1 - v0)
         const float3 e0 = make_float3(a2.x - a0.x, a2.y - a0.y, a2.z - a0.z);
         const float3 e1 = make_float3(a1.x - a0.x, a1.y - a0.y, a1.z - a0.z);

@@ -155,6 +155,55 @@ __global__ void scan_apply_kernel(uint32_t *out, const uint32_t *in, uint32_t n,
     }
 }
 
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// classify_kernel over the scan's tiles, for the orchestrations: the cells
+// of tile b are read and written as kScanItems chunks of kScanThreads
+// consecutive cells, and the tile's sums of verts and occ go to bsum_v[b]
+// and bsum_o[b] (the scan's first pass, without reading the arrays again)
+__global__ void classify_tile_kernel(uint32_t *verts, uint32_t *occ, const float *vol, GridDesc g,
+                                     uint32_t n, float iso, const uint8_t *nverts,
+                                     uint32_t *bsum_v, uint32_t *bsum_o) {
+    constexpr int kWaves = kScanThreads / 64;
+    __shared__ uint32_t part[2][kWaves];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x;
+    uint32_t sv = 0, so = 0;
+#pragma unroll 4
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + k * kScanThreads;
+        if (i < n) {
+            const uint3 p = calc_grid_pos((uint32_t)i, g.shift, g.mask);
+            float f[8];
+            corners(vol, p, g.size, f);
+            const uint32_t nv = nverts[cube_of(f, iso)];
+            verts[i] = nv;
+            occ[i] = nv > 0;
+            sv += nv;
+            so += nv > 0;
+        }
+    }
+    sv = wave_sum(sv);
+    so = wave_sum(so);
+    if ((threadIdx.x & 63) == 0) {
+        part[0][threadIdx.x >> 6] = sv;
+        part[1][threadIdx.x >> 6] = so;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tv = 0, to = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            tv += part[0][w];
+            to += part[1][w];
+        }
+        bsum_v[blockIdx.x] = tv;
+        bsum_o[blockIdx.x] = to;
+    }
+}
+
 __global__ void compact_kernel(uint32_t *comp, const uint32_t *occ, const uint32_t *occ_scan,
                                uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -269,7 +318,7 @@ __global__ void __launch_bounds__(64) generate_kernel(GenArgs a) {
     __shared__ float3 vl[12][64];
     __shared__ uint8_t owner[64 * 15];  // output vertex -> lane
     __shared__ uint32_t s_first[64], s_base[64];
-    __shared__ uint8_t s_cube[64];
+    __shared__ uint4 s_tri[64];  // each lane's triTable row
     const uint32_t lane = threadIdx.x;
     const uint32_t i = blockIdx.x * 64u + lane;
     uint32_t cube = 0, nv = 0, vbase = 0;
@@ -328,17 +377,17 @@ __global__ void __launch_bounds__(64) generate_kernel(GenArgs a) {
     const uint32_t total = __shfl(incl, 63, 64);
     s_first[lane] = first;
     s_base[lane] = vbase;
-    s_cube[lane] = (uint8_t)cube;
+    s_tri[lane] = *(const uint4 *)(a.tri + cube * 16);
     for (uint32_t j = 0; j < nv; ++j) owner[first + j] = (uint8_t)lane;
     __syncthreads();
     for (uint32_t k = lane; k < total; k += 64) {
         const uint32_t o = owner[k];
         const uint32_t j = k - s_first[o];
         const uint32_t t0 = j - j % 3u;  // the triangle's first vertex
-        const uint32_t c = s_cube[o];
-        const float3 a0 = vl[a.tri[c * 16 + t0]][o];
-        const float3 a1 = vl[a.tri[c * 16 + t0 + 1]][o];
-        const float3 a2 = vl[a.tri[c * 16 + t0 + 2]][o];
+        const uint8_t *row = (const uint8_t *)&s_tri[o];
+        const float3 a0 = vl[row[t0]][o];
+        const float3 a1 = vl[row[t0 + 1]][o];
+        const float3 a2 = vl[row[t0 + 2]][o];
         const float3 e0 = make_float3(a1.x - a0.x, a1.y - a0.y, a1.z - a0.z);
         const float3 e1 = make_float3(a2.x - a0.x, a2.y - a0.y, a2.z - a0.z);
         const float3 n = cross3(e0, e1);
@@ -392,6 +441,17 @@ int classify(uint32_t *verts, uint32_t *occ, const float *vol, gc_uint3 s, gc_ui
     return GPUCAD_OK;
 }
 
+// The scan's second and third passes, given the tile sums bsum[nb].
+int scan_finish(uint32_t *out, const uint32_t *in, uint32_t n, uint32_t *bsum, uint32_t nb,
+                hipStream_t st, uint32_t *comp) {
+    hipLaunchKernelGGL(scan_bsum_kernel, dim3(1), dim3(kScanThreads), 0, st, bsum, nb);
+    GC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(kScanThreads), 0, st, out, in, n, bsum,
+                       comp);
+    GC_CHECK_LAUNCH();
+    return GPUCAD_OK;
+}
+
 int scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st,
              uint32_t *comp = nullptr) {
     GC_ARG(out && in, "NULL buffer");
@@ -403,12 +463,7 @@ int scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st,
     uint32_t *bsum = (uint32_t *)ws;
     hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, st, in, n, bsum);
     GC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(scan_bsum_kernel, dim3(1), dim3(kScanThreads), 0, st, bsum, nb);
-    GC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(kScanThreads), 0, st, out, in, n, bsum,
-                       comp);
-    GC_CHECK_LAUNCH();
-    return GPUCAD_OK;
+    return scan_finish(out, in, n, bsum, nb, st, comp);
 }
 
 int compact(uint32_t *comp, const uint32_t *occ, const uint32_t *occ_scan, uint32_t n,
@@ -448,10 +503,12 @@ __global__ void last_pair_kernel(const uint32_t *a, const uint32_t *b, uint32_t 
 // h[0] = a[n-1], h[1] = b[n-1] in one device-to-host copy (the reference
 // copies each with its own cudaMemcpy, Isosurface.cu:424-436, 453-472)
 int read_last_pair(const uint32_t *a, const uint32_t *b, uint32_t n, uint32_t *h,
-                   hipStream_t st) {
-    void *ws = nullptr;
-    int r = scratch(2 * sizeof(uint32_t), &ws);
-    if (r) return r;
+                   hipStream_t st, uint32_t *dev = nullptr) {
+    void *ws = dev;
+    if (!ws) {
+        int r = scratch(2 * sizeof(uint32_t), &ws);
+        if (r) return r;
+    }
     hipLaunchKernelGGL(last_pair_kernel, dim3(1), dim3(64), 0, st, a, b, n, (uint32_t *)ws);
     GC_CHECK_LAUNCH();
     GC_HIP(hipMemcpyAsync(h, ws, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -469,23 +526,35 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
                        float hi1, float lo2, float hi2) {
     GC_ARG(active && total, "host outputs are NULL");
     GC_ARG(n > 0, "numVoxels must be > 0");
+    GC_ARG(d_verts && d_occ && vol && d_occ_scan, "NULL buffer");
     hipStream_t st = 0;
-    int r = classify(d_verts, d_occ, vol, s, sh, m, n, iso, st);
+    int r = check_grid(s, sh, m, n);
     if (r) return r;
+    const DeviceTables *t = nullptr;
+    if ((r = device_tables(&t))) return r;
+    // scratch: the copied-back pair, then the tile sums of occ and verts
+    const uint32_t nb = (uint32_t)(((uint64_t)n + kScanTile - 1) / kScanTile);
+    void *ws = nullptr;
+    if ((r = scratch(16 + 2 * (size_t)nb * sizeof(uint32_t), &ws))) return r;
+    uint32_t *pair = (uint32_t *)ws, *bsum_o = pair + 4, *bsum_v = bsum_o + nb;
+    // classify (src/Isosurface.cu:408-420) with the first pass of both scans
+    const GridDesc g = grid_desc(s, sh, m, gc_float3{1, 1, 1}, gc_float3{0, 0, 0});
+    hipLaunchKernelGGL(classify_tile_kernel, dim3(nb), dim3(kScanThreads), 0, st, d_verts, d_occ,
+                       vol, g, n, iso, t->num_verts, bsum_v, bsum_o);
+    GC_CHECK_LAUNCH();
     // the occupancy scan also writes the compacted voxel list (nothing when
     // no voxel is active, as the reference's early return leaves it)
-    r = scan_u32(d_occ_scan, d_occ, n, st, d_comp);
-    if (r) return r;
+    if ((r = scan_finish(d_occ_scan, d_occ, n, bsum_o, nb, st, d_comp))) return r;
     uint32_t last[2] = {0, 0};
-    if ((r = read_last_pair(d_occ, d_occ_scan, n, last, st))) return r;
+    if ((r = read_last_pair(d_occ, d_occ_scan, n, last, st, pair))) return r;
     *active = last[0] + last[1];
     if (*active == 0) {
         *total = 0;
         return GPUCAD_OK;
     }
-    r = scan_u32(d_verts_scan, d_verts, n, st);
-    if (r) return r;
-    if ((r = read_last_pair(d_verts, d_verts_scan, n, last, st))) return r;
+    GC_ARG(d_verts_scan && d_comp, "NULL buffer");
+    if ((r = scan_finish(d_verts_scan, d_verts, n, bsum_v, nb, st, nullptr))) return r;
+    if ((r = read_last_pair(d_verts, d_verts_scan, n, last, st, pair))) return r;
     *total = last[0] + last[1];
     GenArgs a;
     a.pos = (float4 *)pos;
