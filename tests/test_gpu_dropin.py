@@ -170,6 +170,38 @@ def test_zero_active_voxels(gc, dev):
     assert active == 0 and total == 0
 
 
+def test_zero_active_voxels_leaves_later_buffers_alone(gc, dev):
+    """src/Isosurface.cu:424-436 returns before compactVoxels and the vertex
+    scan when no voxel is active: those buffers keep their contents (and may
+    be NULL), although the occupancy scan runs and writes its own."""
+    n = 19
+    cells = (n - 1) ** 3
+    vol = torch.ones(n ** 3, dtype=torch.float32, device=dev)
+    nrm = torch.zeros_like(vol)
+    gs, sh, mk = grid_args(gc, n, n, n)
+    b = DropinBuffers(cells, 300000, dev)
+    for t in (b.verts_scan, b.comp, b.occ_scan):
+        t.fill_(0x5A5A5A5A)
+    active, total = C.c_uint32(7), C.c_uint32(7)
+    common = (gs, sh, mk, gc.Float3(1, 1, 1), gc.Float3(0, 0, 0), C.byref(active),
+              C.byref(total))
+    gc.check(gc.lib.gpucad_compute_isosurface_lattice(
+        gc.ptr(vol), gc.ptr(b.pos), gc.ptr(b.norm), 0.25, cells, gc.ptr(b.verts),
+        gc.ptr(b.verts_scan), gc.ptr(b.occ), gc.ptr(b.occ_scan), *common, gc.ptr(b.comp), 300000,
+        gc.ptr(nrm), gc.ptr(nrm), 0.2, 0.3, 0.0, 0.0))
+    torch.cuda.synchronize()
+    assert active.value == 0 and total.value == 0
+    assert torch.all(b.verts_scan == 0x5A5A5A5A) and torch.all(b.comp == 0x5A5A5A5A)
+    assert torch.all(b.occ_scan == 0) and torch.all(b.occ == 0) and torch.all(b.verts == 0)
+    # the same call with NULL for the buffers the reference does not reach
+    gc.check(gc.lib.gpucad_compute_isosurface_lattice(
+        gc.ptr(vol), gc.ptr(b.pos), gc.ptr(b.norm), 0.25, cells, gc.ptr(b.verts), None,
+        gc.ptr(b.occ), gc.ptr(b.occ_scan), *common, None, 300000, gc.ptr(nrm), gc.ptr(nrm), 0.2,
+        0.3, 0.0, 0.0))
+    torch.cuda.synchronize()
+    assert active.value == 0 and total.value == 0
+
+
 @pytest.mark.parametrize("n", [1, 2, 4095, 4096, 4097, 1_000_003, 9_000_000])
 def test_exclusive_scan_u32(gc, orc, dev, n):
     rng = np.random.default_rng(n)
