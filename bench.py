@@ -325,6 +325,55 @@ def bench_dropin(n=120, period=30.0, runs=5, cpu=True):
     ms_iso = 1e3 * statistics.median(t_iso)
     tris = total.value // 3
     nbytes = 3 * 4 * npts + 96 * tris
+    lat_counts = (active.value, total.value)
+
+    # boolean meshing (Isosurface::computeIsosurface after classify_copy_Voxel,
+    # src/main.cu:3304-3465): a sphere primitive's grid_points united with the
+    # lattice's band, "dynamic" mode (corner inside: F < 0 or 0.2 < L < 0.3)
+    flags = 1 | 16
+    idx = torch.arange(npts, device=dev)
+    zc, yc, xc = idx // (nx * ny), (idx // nx) % ny, idx % nx
+    sph = (torch.sqrt((xc - 0.5 * (nx - 1)) ** 2 + (yc - 0.5 * (ny - 1)) ** 2 +
+                      (zc - 0.5 * (nz - 1)) ** 2) - 0.3 * nx).float().contiguous()
+    del idx, zc, yc, xc
+    g0 = torch.zeros((npts, 4), dtype=torch.int32, device=dev)  # grid_points {val, t_x, t_y, t_z}
+    g0[:, 0] = torch.where(sph < 0, -1, 1).int()
+    d_g = torch.empty_like(g0)
+    nfac = C.c_uint64(0)
+
+    def copy_pass():
+        gc.check(gc.lib.gpucad_classify_copy_voxel(gc.ptr(d_g), gc.ptr(sph), gc.ptr(nrm), gs, 0.2,
+                                                   0.3, 0.0, flags))
+
+    def boolean():
+        gc.check(gc.lib.gpucad_compute_isosurface(
+            gc.ptr(pos), gc.ptr(norm), 0.0, cells, gc.ptr(u[0]), gc.ptr(u[1]), gc.ptr(u[2]),
+            gc.ptr(u[3]), gs, sh, mk, vs, ce, C.byref(active), C.byref(total), gc.ptr(u[4]),
+            max_verts, gc.ptr(d_g), gc.ptr(sph), gc.ptr(nrm), 0.2, 0.3, flags, C.byref(nfac)))
+
+    t_copy, t_bool = [], []
+    for r in range(runs + 1):
+        d_g.copy_(g0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        copy_pass()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        boolean()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if r:
+            t_copy.append(t1 - t0)
+            t_bool.append(t2 - t1)
+    ms_copy = 1e3 * statistics.median(t_copy)
+    ms_bool = 1e3 * statistics.median(t_bool)
+    btris = total.value // 3
+    bool_counts = (active.value, total.value)
+    out_bool = {"workload": "computeIsosurface: sphere primitive (r = 0.3 N, grid_points) united "
+                            "with the lattice band 0.20-0.30, dynamic mode, %d^3" % nx,
+                "classify_copy_ms": round(ms_copy, 3), "isosurface_ms": round(ms_bool, 3),
+                "active_voxels": active.value, "total_verts": total.value,
+                "mtri_per_s": round(btris / (ms_bool * 1e-3) / 1e6, 1)}
     out = {"workload": "drop-in computeIsosurface_lattice of the synthesised b lattice "
                        "(period %g, fine %d^3), band 0.20-0.30, isoValue 0.25" % (period, nx),
            "grid": [nx, ny, nz], "runs": runs,
@@ -332,7 +381,7 @@ def bench_dropin(n=120, period=30.0, runs=5, cpu=True):
            "isosurface_ms": round(ms_iso, 3),
            "ms": round(ms_norm + ms_iso, 3),
            "gvoxels_per_s": round(npts / ((ms_norm + ms_iso) * 1e-3) / 1e9, 3),
-           "active_voxels": active.value, "total_verts": total.value,
+           "active_voxels": lat_counts[0], "total_verts": lat_counts[1],
            "mtri_per_s": round(tris / (ms_iso * 1e-3) / 1e6, 1),
            "isosurface_GBs": round(nbytes / (ms_iso * 1e-3) / 1e9, 1),
            "isosurface_frac_hbm": round(nbytes / (ms_iso * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
@@ -350,10 +399,23 @@ def bench_dropin(n=120, period=30.0, runs=5, cpu=True):
         t2 = time.perf_counter()
         out["cpu_oracle"] = {"threads": 1, "normalise_four_ms": round(1e3 * (t1 - t0), 1),
                              "isosurface_ms": round(1e3 * (t2 - t1), 1),
-                             "same_counts": (ref["active"], ref["total"]) == (active.value,
-                                                                              total.value)}
+                             "same_counts": (ref["active"], ref["total"]) == lat_counts}
         out["speedup_vs_cpu"] = round(1e3 * (t2 - t0) / (ms_norm + ms_iso), 1)
-    del pos, norm, u
+        s_np, l_np = sph.cpu().numpy(), nrm.cpu().numpy()
+        g_np = np.zeros(npts, orc.GRID_POINT)
+        g_np["val"] = np.where(s_np < 0, -1, 1)
+        t0 = time.perf_counter()
+        g_ref = orc.bool_copy(flags, g_np, s_np, l_np, nx, ny, nz, 0.0, 0.2, 0.3)
+        t1 = time.perf_counter()
+        b_ref = orc.bool_compute_isosurface(flags, g_ref, s_np, l_np, nx, ny, nz, 0.0, 0.2, 0.3,
+                                            voxel_size=(0.5, 0.5, 0.5), max_verts=max_verts)
+        t2 = time.perf_counter()
+        out_bool["cpu_oracle"] = {"threads": 1, "classify_copy_ms": round(1e3 * (t1 - t0), 1),
+                                  "isosurface_ms": round(1e3 * (t2 - t1), 1),
+                                  "same_counts": (b_ref["active"], b_ref["total"]) == bool_counts}
+        out_bool["speedup_vs_cpu"] = round(1e3 * (t2 - t0) / (ms_copy + ms_bool), 1)
+    out["boolean"] = out_bool
+    del pos, norm, u, d_g, g0
     torch.cuda.empty_cache()
     return out
 
