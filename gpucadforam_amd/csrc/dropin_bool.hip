@@ -103,19 +103,25 @@ struct Vols {
 
 // which volumes a mode reads (fixed takes precedence over dynamic, as in the
 // reference's if / else-if chains)
-__host__ __device__ inline void needs(uint32_t flags, bool *f, bool *d, bool *l) {
-    *f = *d = *l = false;
+// (returned by value: flags written through pointers were kept in private
+// memory by the kernels)
+struct Needs {
+    bool f, d, l;
+};
+__host__ __device__ __forceinline__ Needs needs(uint32_t flags) {
+    Needs n = {false, false, false};
     if (flags & GC_BOOL_MAKE_REGION) {
-        *f = true;
+        n.f = true;
     } else if (flags & (GC_BOOL_UNION | GC_BOOL_DIFF | GC_BOOL_INTERSECT)) {
         if (flags & GC_BOOL_FIXED) {
-            *d = *l = true;
+            n.d = n.l = true;
         } else if (flags & GC_BOOL_DYNAMIC) {
-            *f = *l = true;
+            n.f = n.l = true;
         } else {
-            *f = *d = true;
+            n.f = n.d = true;
         }
     }
+    return n;
 }
 
 __device__ __forceinline__ uint32_t cube_index(uint32_t flags, const Vols &v, const uint32_t idx[8],
@@ -137,8 +143,8 @@ __global__ void bool_classify_kernel(uint32_t *verts, uint32_t *occ, Vols v, BGr
                                      const uint8_t *nverts) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    bool nf, nd, nl;
-    needs(flags, &nf, &nd, &nl);
+    const Needs nn = needs(flags);
+    const bool nf = nn.f, nd = nn.d, nl = nn.l;
     const uint3 p = calc_grid_pos(i, g.shift, g.mask);
     uint32_t idx[8];
     corner_index(p, g.size, idx);
@@ -317,8 +323,8 @@ __global__ void __launch_bounds__(64) bool_generate_kernel(BGenArgs a) {
         v[5] = make_float3(p.x + vs.x, p.y + 0.f, p.z + vs.z);
         v[6] = make_float3(p.x + vs.x, p.y + vs.y, p.z + vs.z);
         v[7] = make_float3(p.x + 0.f, p.y + vs.y, p.z + vs.z);
-        bool nf, nd, nl;
-        needs(a.flags, &nf, &nd, &nl);
+        const Needs nn = needs(a.flags);
+        const bool nf = nn.f, nd = nn.d, nl = nn.l;
         uint32_t idx[8];
         corner_index(gp, a.g.size, idx);
         cube = cube_index(a.flags, a.v, idx, a.iso, a.lo, a.hi, nf, nd, nl);
@@ -397,8 +403,8 @@ BGrid bgrid(gc_uint3 s, gc_uint3 sh, gc_uint3 m, gc_float3 vs, gc_float3 c) {
 }
 
 int check_vols(uint32_t flags, const void *fixed, const float *dyn, const float *lat) {
-    bool nf, nd, nl;
-    needs(flags, &nf, &nd, &nl);
+    const Needs nn = needs(flags);
+    const bool nf = nn.f, nd = nn.d, nl = nn.l;
     GC_ARG(!nf || fixed, "primitive_fixed is NULL but the mode reads it");
     GC_ARG(!nd || dyn, "primitive_dynamic is NULL but the mode reads it");
     GC_ARG(!nl || lat, "lattice_field is NULL but the mode reads it");
