@@ -421,7 +421,7 @@ int batch_create(const gc_lattice_params *grids, int n, const gc_batch_options *
     plan_segments(grids, n, b->field_off.data(), sign_off.data(), o->iso, o->unit_domain,
                   b->plan);
     const uint64_t nb = b->plan.nb;
-    // ---- one arena for everything but the brick queue and the meshes
+    // ---- one arena for everything but the brick list and the meshes
     struct Part {
         uint64_t off, bytes;
     };
@@ -462,7 +462,7 @@ int batch_create(const gc_lattice_params *grids, int n, const gc_batch_options *
         hipMemcpy(b->d.blk2seg, b->plan.blk2seg.data(), nb * 4, hipMemcpyHostToDevice) !=
             hipSuccess)
         return bail(fail(GPUCAD_E_HIP, "batch upload failed"));
-    // ---- brick queue big enough for every candidate shape
+    // ---- brick list big enough for every candidate shape
     uint64_t most = 0;
     for (int bx : {64, 32, 16, 8})
         for (int lz : {1, 8, 16, 32, o->brick_lz > 0 ? o->brick_lz : 16}) {
@@ -474,7 +474,7 @@ int batch_create(const gc_lattice_params *grids, int n, const gc_batch_options *
     b->items_cap = most;
     if (hipMalloc(&b->items_buf, most * 8) != hipSuccess) {
         b->items_buf = nullptr;
-        return bail(fail(GPUCAD_E_OOM, "brick queue allocation failed"));
+        return bail(fail(GPUCAD_E_OOM, "brick list allocation failed"));
     }
     b->d.items = b->items_buf;
     if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||

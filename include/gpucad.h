@@ -175,9 +175,9 @@ int gpucad_extract(const gc_extract_args *a, gc_mesh_counts *counts, void *strea
 /* (no counterpart in the reference)                                        */
 /* ------------------------------------------------------------------------ */
 /* A batch owns the fields, sign words, extraction workspace and mesh buffers
- * of its grids.  gpucad_batch_run evaluates every field and extracts every
- * mesh with persistent workgroups fed by device queues of bricks (field) and
- * word blocks (extraction); with use_graph the whole run is one replay of a
+ * of its grids.  gpucad_batch_run evaluates every field (one workgroup per
+ * brick of any grid) and extracts every mesh (one workgroup per block of
+ * sign words of any grid); with use_graph the whole run is one replay of a
  * captured HIP graph.  Each grid's mesh is bit-identical to what
  * gpucad_lattice_field + gpucad_extract produce for it alone. */
 typedef struct gc_batch gc_batch;
