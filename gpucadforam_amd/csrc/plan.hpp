@@ -1,5 +1,5 @@
 // Launch plans of the many-grid forms (batched: batch.hip, served: server.cpp):
-// which grids, where their fields and sign words live, the brick queue of the
+// which grids, where their fields and sign words live, the brick list of the
 // field launches and the block -> grid table of the extraction launches.
 #pragma once
 
