@@ -18,6 +18,7 @@ struct DeviceState {
     bool tables_ready = false;
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    uint32_t *host_words = nullptr;  // pinned host memory that kernels write
 };
 
 DeviceState g_dev[kMaxDevices];
@@ -60,6 +61,23 @@ int device_tables(const DeviceTables **out) {
         ds.tables_ready = true;
     }
     *out = &ds.tables;
+    return GPUCAD_OK;
+}
+
+int host_words(uint32_t **out) {
+    int dev = 0;
+    int st = current_device(&dev);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceState &ds = g_dev[dev];
+    if (!ds.host_words) {
+        void *p = nullptr;
+        hipError_t e = hipHostMalloc(&p, 64, hipHostMallocCoherent);
+        if (e != hipSuccess)
+            return fail(GPUCAD_E_OOM, "pinned host allocation failed: %s", hipGetErrorString(e));
+        ds.host_words = (uint32_t *)p;
+    }
+    *out = ds.host_words;
     return GPUCAD_OK;
 }
 
@@ -152,6 +170,10 @@ int gpucad_shutdown(void) {
         GC_HIP(hipFree(ds.scratch));
         ds.scratch = nullptr;
         ds.scratch_bytes = 0;
+    }
+    if (ds.host_words) {
+        GC_HIP(hipHostFree(ds.host_words));
+        ds.host_words = nullptr;
     }
     return GPUCAD_OK;
 }

@@ -500,19 +500,19 @@ __global__ void last_pair_kernel(const uint32_t *a, const uint32_t *b, uint32_t 
     }
 }
 
-// h[0] = a[n-1], h[1] = b[n-1] in one device-to-host copy (the reference
-// copies each with its own cudaMemcpy, Isosurface.cu:424-436, 453-472)
+// h[0] = a[n-1], h[1] = b[n-1]: a one-thread kernel writes them into pinned
+// host memory, read after the stream synchronisation (the reference copies
+// each with its own cudaMemcpy, Isosurface.cu:424-436, 453-472)
 int read_last_pair(const uint32_t *a, const uint32_t *b, uint32_t n, uint32_t *h,
-                   hipStream_t st, uint32_t *dev = nullptr) {
-    void *ws = dev;
-    if (!ws) {
-        int r = scratch(2 * sizeof(uint32_t), &ws);
-        if (r) return r;
-    }
-    hipLaunchKernelGGL(last_pair_kernel, dim3(1), dim3(64), 0, st, a, b, n, (uint32_t *)ws);
+                   hipStream_t st) {
+    uint32_t *hw = nullptr;
+    int r = host_words(&hw);
+    if (r) return r;
+    hipLaunchKernelGGL(last_pair_kernel, dim3(1), dim3(64), 0, st, a, b, n, hw);
     GC_CHECK_LAUNCH();
-    GC_HIP(hipMemcpyAsync(h, ws, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     GC_HIP(hipStreamSynchronize(st));
+    h[0] = hw[0];
+    h[1] = hw[1];
     return GPUCAD_OK;
 }
 
@@ -532,11 +532,11 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
     if (r) return r;
     const DeviceTables *t = nullptr;
     if ((r = device_tables(&t))) return r;
-    // scratch: the copied-back pair, then the tile sums of occ and verts
+    // scratch: the tile sums of occ and verts
     const uint32_t nb = (uint32_t)(((uint64_t)n + kScanTile - 1) / kScanTile);
     void *ws = nullptr;
-    if ((r = scratch(16 + 2 * (size_t)nb * sizeof(uint32_t), &ws))) return r;
-    uint32_t *pair = (uint32_t *)ws, *bsum_o = pair + 4, *bsum_v = bsum_o + nb;
+    if ((r = scratch(2 * (size_t)nb * sizeof(uint32_t), &ws))) return r;
+    uint32_t *bsum_o = (uint32_t *)ws, *bsum_v = bsum_o + nb;
     // classify (src/Isosurface.cu:408-420) with the first pass of both scans
     const GridDesc g = grid_desc(s, sh, m, gc_float3{1, 1, 1}, gc_float3{0, 0, 0});
     hipLaunchKernelGGL(classify_tile_kernel, dim3(nb), dim3(kScanThreads), 0, st, d_verts, d_occ,
@@ -546,7 +546,7 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
     // no voxel is active, as the reference's early return leaves it)
     if ((r = scan_finish(d_occ_scan, d_occ, n, bsum_o, nb, st, d_comp))) return r;
     uint32_t last[2] = {0, 0};
-    if ((r = read_last_pair(d_occ, d_occ_scan, n, last, st, pair))) return r;
+    if ((r = read_last_pair(d_occ, d_occ_scan, n, last, st))) return r;
     *active = last[0] + last[1];
     if (*active == 0) {
         *total = 0;
@@ -554,7 +554,7 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
     }
     GC_ARG(d_verts_scan && d_comp, "NULL buffer");
     if ((r = scan_finish(d_verts_scan, d_verts, n, bsum_v, nb, st, nullptr))) return r;
-    if ((r = read_last_pair(d_verts, d_verts_scan, n, last, st, pair))) return r;
+    if ((r = read_last_pair(d_verts, d_verts_scan, n, last, st))) return r;
     *total = last[0] + last[1];
     GenArgs a;
     a.pos = (float4 *)pos;

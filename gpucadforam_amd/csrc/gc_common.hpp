@@ -61,12 +61,15 @@ int check_dropin_grid(gc_uint3 s, gc_uint3 sh, gc_uint3 m, uint32_t n);
 // comp[out[i]] = i where in[i] != 0.
 int dropin_scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st,
                     uint32_t *comp = nullptr);
-// h[0] = a[n-1], h[1] = b[n-1] in one device-to-host copy, after the work on st
+// h[0] = a[n-1], h[1] = b[n-1] through pinned host memory, after the work on st
 int dropin_read_last_pair(const uint32_t *a, const uint32_t *b, uint32_t n, uint32_t *h,
                           hipStream_t st);
 
 // Scratch memory owned by the library for drop-in calls (grown on demand,
 // one arena per device, never shrunk).
 int scratch(size_t bytes, void **out);
+// 16 words of pinned host memory per device, written by kernels and read by
+// the host after a stream synchronisation (allocated on first use).
+int host_words(uint32_t **out);
 
 }  // namespace gc
