@@ -374,3 +374,33 @@ def test_compat_classes_drive_the_library(gc, orc, dev, tmp_path):
         gc.check(gc.lib.gpucad_file_write_obj(gc.ptr(b.pos), t2, path.encode()))
         assert open(path, "rb").read() == \
             open(str(tmp_path / ("unit_lattice_%d.obj" % ltype)), "rb").read()
+
+
+def test_file_write_obj_known_answer(gc, dev, tmp_path):
+    """The product writer on the hand-worked .obj input (the literal expected
+    file is in tests/test_oracle_pins.py)."""
+    from tests.test_oracle_pins import OBJ_EXPECTED, obj_input
+    pos = torch.from_numpy(obj_input()).to(dev)
+    path = str(tmp_path / "kat.obj")
+    gc.check(gc.lib.gpucad_file_write_obj(gc.ptr(pos), pos.shape[0], path.encode()))
+    assert open(path).read() == OBJ_EXPECTED
+
+
+@pytest.mark.parametrize("second_band", [False, True])
+@pytest.mark.parametrize("nz,corner_z", [(2, 0), (3, 1)])
+def test_band_equal_values_branch(gc, dev, nz, corner_z, second_band):
+    """Equal normalised values outside both band levels on an indicator 1/0
+    (2/0) edge: t = 1 when p0.z == 0, else 0; hand-worked vertices and normals."""
+    from tests.test_oracle_pins import (BAND_ONE_CELL, BAND_TWO_CELLS, band_case,
+                                        expected_band)
+    ind, k1, k2 = (torch.from_numpy(a).to(dev) for a in band_case(nz, corner_z, second_band))
+    if second_band:
+        b, active, total = run_dropin(gc, "lattice", ind, 2, 2, nz, 0.25, vol1=k1, vol2=k2,
+                                      lo1=0.2, hi1=0.3, lo2=0.6, hi2=0.7)
+    else:
+        b, active, total = run_dropin(gc, "latticeone", ind, 2, 2, nz, 0.25, vol1=k1, lo1=0.2,
+                                      hi1=0.3)
+    pos, nrm = expected_band(BAND_ONE_CELL if nz == 2 else BAND_TWO_CELLS)
+    assert total == len(pos)
+    assert np.array_equal(to_np(b.pos)[:total, :3], pos)
+    assert np.array_equal(to_np(b.norm)[:total, :3], nrm)
