@@ -1,5 +1,6 @@
 // Library-level C-ABI: version, status strings, thread-local error messages,
-// device initialisation (marching-cubes tables) and the drop-in scratch arena.
+// device initialisation (marching-cubes tables, pinned host words, the fixed
+// scratch of the allocation-free drop-in) and the drop-in scratch arena.
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
@@ -19,6 +20,9 @@ struct DeviceState {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     uint32_t *host_words = nullptr;  // pinned host memory that kernels write
+    // held by every synchronous entry point that uses the arena or the host
+    // words, so calls from several host threads on one device take turns
+    std::recursive_mutex call_mu;
 };
 
 DeviceState g_dev[kMaxDevices];
@@ -81,7 +85,35 @@ int host_words(uint32_t **out) {
     return GPUCAD_OK;
 }
 
-int scratch(size_t bytes, void **out) {
+int tables_if_ready(const DeviceTables **out) {
+    int dev = 0;
+    int st = current_device(&dev);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceState &ds = g_dev[dev];
+    if (!ds.tables_ready) return fail(GPUCAD_E_ARG, "gpucad_init was not called on device %d", dev);
+    *out = &ds.tables;
+    return GPUCAD_OK;
+}
+
+int host_words_if_ready(uint32_t **out) {
+    int dev = 0;
+    int st = current_device(&dev);
+    if (st) return st;
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceState &ds = g_dev[dev];
+    if (!ds.host_words) return fail(GPUCAD_E_ARG, "gpucad_init was not called on device %d", dev);
+    *out = ds.host_words;
+    return GPUCAD_OK;
+}
+
+std::unique_lock<std::recursive_mutex> lock_device_calls() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+    return std::unique_lock<std::recursive_mutex>(g_dev[dev].call_mu);
+}
+
+int scratch_arena(size_t bytes, void **out) {
     int dev = 0;
     int st = current_device(&dev);
     if (st) return st;
@@ -148,7 +180,10 @@ int gpucad_init(int device) {
     GC_ARG(device >= 0 && device < n, "device %d not present (%d devices)", device, n);
     GC_HIP(hipSetDevice(device));
     const gc::DeviceTables *t = nullptr;
-    return gc::device_tables(&t);
+    int st = gc::device_tables(&t);
+    if (st) return st;
+    uint32_t *hw = nullptr;
+    return gc::host_words(&hw);
 }
 
 int gpucad_shutdown(void) {
@@ -156,6 +191,7 @@ int gpucad_shutdown(void) {
     int dev = 0;
     int st = gc::current_device(&dev);
     if (st) return st;
+    auto calls = gc::lock_device_calls();
     std::lock_guard<std::mutex> lk(gc::g_mu);
     gc::DeviceState &ds = gc::g_dev[dev];
     GC_HIP(hipDeviceSynchronize());

@@ -525,6 +525,7 @@ int gpucad_compute_isosurface(gc_float4 *pos, gc_float4 *norm, float iso, uint32
                               const float *primitive_dynamic, const float *lattice_field,
                               float iso1, float iso2, uint32_t flags, uint64_t *nfacets) {
     clear_error();
+    auto calls = lock_device_calls();  // shared scratch arena and host words
     GC_ARG(active_voxels && total_verts && nfacets, "host outputs are NULL");
     GC_ARG(num_voxels > 0, "numVoxels must be > 0");
     GC_ARG(pos && norm && d_comp_voxel_array && d_voxel_verts_scan && d_voxel_occupied_scan,

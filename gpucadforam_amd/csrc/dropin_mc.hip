@@ -458,7 +458,7 @@ int scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st,
     if (n == 0) return GPUCAD_OK;
     uint32_t nb = (uint32_t)(((uint64_t)n + kScanTile - 1) / kScanTile);
     void *ws = nullptr;
-    int r = scratch((size_t)nb * sizeof(uint32_t), &ws);
+    int r = scratch_arena((size_t)nb * sizeof(uint32_t), &ws);
     if (r) return r;
     uint32_t *bsum = (uint32_t *)ws;
     hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, st, in, n, bsum);
@@ -535,7 +535,7 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
     // scratch: the tile sums of occ and verts
     const uint32_t nb = (uint32_t)(((uint64_t)n + kScanTile - 1) / kScanTile);
     void *ws = nullptr;
-    if ((r = scratch(2 * (size_t)nb * sizeof(uint32_t), &ws))) return r;
+    if ((r = scratch_arena(2 * (size_t)nb * sizeof(uint32_t), &ws))) return r;
     uint32_t *bsum_o = (uint32_t *)ws, *bsum_v = bsum_o + nb;
     // classify (src/Isosurface.cu:408-420) with the first pass of both scans
     const GridDesc g = grid_desc(s, sh, m, gc_float3{1, 1, 1}, gc_float3{0, 0, 0});
@@ -613,6 +613,7 @@ int gpucad_classify_voxel_lattice_new(uint32_t *voxel_verts, uint32_t *voxel_occ
 
 int gpucad_exclusive_scan_u32(uint32_t *output, const uint32_t *input, uint32_t n) {
     clear_error();
+    auto calls = lock_device_calls();  // shared scratch arena and host words
     int r = scan_u32(output, input, n, 0);
     if (r) return r;
     GC_HIP(hipStreamSynchronize(0));
@@ -726,6 +727,7 @@ int gpucad_compute_isosurface_lattice(const float *vol, gc_float4 *pos, gc_float
                                       const float *vol_one, const float *vol_two, float isovalue1,
                                       float isovalue2, float iso1, float iso2) {
     clear_error();
+    auto calls = lock_device_calls();  // shared scratch arena and host words
     return compute_isosurface<GEN_BAND2>(
         vol, pos, norm, iso, num_voxels, d_voxel_verts, d_voxel_verts_scan, d_voxel_occupied,
         d_voxel_occupied_scan, grid_size, grid_size_shift, grid_size_mask, voxel_size, grid_center,
@@ -743,6 +745,7 @@ int gpucad_compute_isosurface_latticeone(const float *vol, gc_float4 *pos, gc_fl
                                          uint32_t *d_comp_voxel_array, uint32_t max_verts,
                                          const float *vol_one, float isovalue1, float isovalue2) {
     clear_error();
+    auto calls = lock_device_calls();  // shared scratch arena and host words
     return compute_isosurface<GEN_BAND1>(
         vol, pos, norm, iso, num_voxels, d_voxel_verts, d_voxel_verts_scan, d_voxel_occupied,
         d_voxel_occupied_scan, grid_size, grid_size_shift, grid_size_mask, voxel_size, grid_center,
@@ -759,6 +762,7 @@ int gpucad_compute_isosurface_standard(const float *vol, gc_float4 *pos, gc_floa
                                        uint32_t *active_voxels, uint32_t *total_verts,
                                        uint32_t *d_comp_voxel_array, uint32_t max_verts) {
     clear_error();
+    auto calls = lock_device_calls();  // shared scratch arena and host words
     return compute_isosurface<GEN_STD>(
         vol, pos, norm, iso, num_voxels, d_voxel_verts, d_voxel_verts_scan, d_voxel_occupied,
         d_voxel_occupied_scan, grid_size, grid_size_shift, grid_size_mask, voxel_size, grid_center,

@@ -576,12 +576,13 @@ int gpucad_distance_from_line(float *data, gc_float3 center, gc_float3 axis, flo
 int gpucad_normalise_four(const float *in, float *indicator, float *normalised, uint64_t size,
                           int32_t nx, int32_t ny, int32_t nz, float iso1, float iso2) {
     clear_error();
+    auto calls = lock_device_calls();  // shared scratch arena
     GC_ARG(in && indicator && normalised, "NULL buffer");
     GC_ARG(nx >= 1 && ny >= 1 && nz >= 1, "grid sizes must be positive");
     GC_ARG(size == (uint64_t)nx * ny * nz, "size must equal nx*ny*nz");
     const int nparts = 1024;
     void *ws = nullptr;
-    int st = scratch(nparts * sizeof(float2), &ws);
+    int st = scratch_arena(nparts * sizeof(float2), &ws);
     if (st) return st;
     float2 *part = (float2 *)ws;
     hipLaunchKernelGGL(minmax_partial_kernel, dim3(nparts), dim3(256), 0, 0, in, size, part);

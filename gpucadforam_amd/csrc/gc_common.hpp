@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "../../include/gpucad.h"
@@ -66,10 +67,21 @@ int dropin_read_last_pair(const uint32_t *a, const uint32_t *b, uint32_t n, uint
                           hipStream_t st);
 
 // Scratch memory owned by the library for drop-in calls (grown on demand,
-// one arena per device, never shrunk).
-int scratch(size_t bytes, void **out);
+// one arena per device, never shrunk).  Callers hold lock_device_calls().
+int scratch_arena(size_t bytes, void **out);
 // 16 words of pinned host memory per device, written by kernels and read by
-// the host after a stream synchronisation (allocated on first use).
+// the host after a stream synchronisation (allocated on first use or by
+// gpucad_init).  Words 0-1 serve the drop-in orchestrations.
 int host_words(uint32_t **out);
+
+// The same tables and host words, without allocating: GPUCAD_E_ARG when
+// gpucad_init has not prepared them on the current device.
+int tables_if_ready(const DeviceTables **out);
+int host_words_if_ready(uint32_t **out);
+
+// Per-device lock held by the synchronous entry points that use the scratch
+// arena or the host words for the whole call (recursive: entry points call
+// each other's pieces).
+std::unique_lock<std::recursive_mutex> lock_device_calls();
 
 }  // namespace gc

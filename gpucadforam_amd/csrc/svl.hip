@@ -492,7 +492,7 @@ int cg_solve(const float *b, float *x, int m, int nx, int ny, int nz, int max_it
     const uint64_t slot = (vec + 255) & ~uint64_t(255);  // 256-byte aligned vectors
     const uint64_t bytes = 5 * slot + (uint64_t)m * a.G * 8 * 3 + kMaxRhs * sizeof(CgState) + 1024;
     void *ws = nullptr;
-    int rc = scratch(bytes, &ws);
+    int rc = scratch_arena(bytes, &ws);
     if (rc) return rc;
     char *w = (char *)ws;
     a.phi = (float *)w;
@@ -859,10 +859,11 @@ int gpucad_svl_angle_data(float *d_theta, int32_t nx, int32_t ny, int32_t nz, fl
 
 static int rescale(const float *in, float *out, uint64_t size, float a1, float b1, int affine) {
     clear_error();
+    auto calls = lock_device_calls();  // shared scratch arena
     GC_ARG(in && out, "NULL buffer");
     GC_ARG(size > 0, "size must be positive");
     void *ws = nullptr;
-    int st = scratch(1024 * sizeof(float2), &ws);
+    int st = scratch_arena(1024 * sizeof(float2), &ws);
     if (st) return st;
     float2 *part = (float2 *)ws;
     st = enqueue_minmax0(in, size, part, 0);
@@ -930,6 +931,7 @@ int gpucad_svl_matvec(const float *d_d, float *d_q, int32_t nx, int32_t ny, int3
 int gpucad_svl_cg(float *d_phi, int32_t nx, int32_t ny, int32_t nz, int32_t iter, float end_res,
                   int32_t *final_iter, float *final_res) {
     clear_error();
+    auto calls = lock_device_calls();  // shared scratch arena
     GC_ARG(d_phi, "NULL buffer");
     int st = check_dims(nx, ny, nz, 4);
     if (st) return st;
@@ -954,11 +956,12 @@ int gpucad_svl_axpby(float *d_v, float a1, const float *d_w, float a2, uint64_t 
 
 int gpucad_svl_dot(float *d_result, const float *d_a, const float *d_b, uint64_t n) {
     clear_error();
+    auto calls = lock_device_calls();  // shared scratch arena
     GC_ARG(d_result && d_a && d_b, "NULL buffer");
     const int G = (int)std::max<uint64_t>(1, std::min<uint64_t>((n + kCgThreads - 1) / kCgThreads,
                                                                 kCgBlocks));
     void *ws = nullptr;
-    int st = scratch(G * sizeof(double), &ws);
+    int st = scratch_arena(G * sizeof(double), &ws);
     if (st) return st;
     hipLaunchKernelGGL(dot_partial_kernel, dim3(G), dim3(kCgThreads), 0, 0, d_a, d_b, n,
                        (double *)ws);
@@ -1031,6 +1034,7 @@ int gpucad_svl_accumulate(float *d_svl, const gc_float2 *d_grating, uint64_t n, 
 int gpucad_svl_field(const gc_svl_params *p, const float *d_period, const gc_float2 *d_coeffs,
                      float *d_svl, gc_svl_result *result, void *stream) {
     clear_error();
+    auto calls = lock_device_calls();  // shared scratch arena
     int st = check_params(p, d_period);
     if (st) return st;
     GC_ARG(d_coeffs && d_svl, "NULL buffer");
@@ -1095,6 +1099,7 @@ int gpucad_svl_field(const gc_svl_params *p, const float *d_period, const gc_flo
 int gpucad_unit_spectrum(const float *d_cell, int32_t nx, int32_t ny, int32_t nz, int32_t range,
                          gc_float2 *d_coeffs) {
     clear_error();
+    auto calls = lock_device_calls();  // shared scratch arena
     GC_ARG(d_cell && d_coeffs, "NULL buffer");
     int st = check_dims(nx, ny, nz, 1);
     if (st) return st;
@@ -1115,7 +1120,7 @@ int gpucad_unit_spectrum(const float *d_cell, int32_t nx, int32_t ny, int32_t nz
         }
     }
     void *ws = nullptr;
-    st = scratch(tw.size() * sizeof(double2), &ws);
+    st = scratch_arena(tw.size() * sizeof(double2), &ws);
     if (st) return st;
     GC_HIP(hipMemcpy(ws, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(dft_kernel, dim3((unsigned)(side * side * side)), dim3(256), 0, 0, d_cell,
