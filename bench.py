@@ -24,6 +24,11 @@ import statistics
 import sys
 import time
 
+# thread placement of the CPU baseline's OpenMP runtime (read when the oracle
+# library loads, so set before anything imports it)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 import torch
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -397,7 +402,7 @@ def bench_dropin(n=120, period=30.0, runs=5, cpu=True):
                                      vol2=np.zeros_like(r_nrm), lo1=0.2, hi1=0.3,
                                      max_verts=max_verts)
         t2 = time.perf_counter()
-        out["cpu_oracle"] = {"threads": 1, "normalise_four_ms": round(1e3 * (t1 - t0), 1),
+        out["cpu_oracle"] = {"threads": orc.threads(), "normalise_four_ms": round(1e3 * (t1 - t0), 1),
                              "isosurface_ms": round(1e3 * (t2 - t1), 1),
                              "same_counts": (ref["active"], ref["total"]) == lat_counts}
         out["speedup_vs_cpu"] = round(1e3 * (t2 - t0) / (ms_norm + ms_iso), 1)
@@ -410,7 +415,7 @@ def bench_dropin(n=120, period=30.0, runs=5, cpu=True):
         b_ref = orc.bool_compute_isosurface(flags, g_ref, s_np, l_np, nx, ny, nz, 0.0, 0.2, 0.3,
                                             voxel_size=(0.5, 0.5, 0.5), max_verts=max_verts)
         t2 = time.perf_counter()
-        out_bool["cpu_oracle"] = {"threads": 1, "classify_copy_ms": round(1e3 * (t1 - t0), 1),
+        out_bool["cpu_oracle"] = {"threads": orc.threads(), "classify_copy_ms": round(1e3 * (t1 - t0), 1),
                                   "isosurface_ms": round(1e3 * (t2 - t1), 1),
                                   "same_counts": (b_ref["active"], b_ref["total"]) == bool_counts}
         out_bool["speedup_vs_cpu"] = round(1e3 * (t2 - t0) / (ms_copy + ms_bool), 1)
@@ -418,6 +423,17 @@ def bench_dropin(n=120, period=30.0, runs=5, cpu=True):
     del pos, norm, u, d_g, g0
     torch.cuda.empty_cache()
     return out
+
+
+def cpu_model():
+    """The host CPU's model name (what lscpu prints as "Model name")."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(p, slab, threads, budget_s=12.0):
@@ -446,6 +462,10 @@ def cpu_baseline(p, slab, threads, budget_s=12.0):
         "value": round(pts / tot / 1e9, 5),
         "unit": "Gvoxels/s",
         "cores": nthreads,
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
+        "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+        "build": "g++ -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off (oracle/Makefile)",
         "kind": "port",
         "sample": "median of %d z-slabs of %d planes of the %dx%dx%d workload (%d points each, "
                   "%.1f s total): oracle field %.3f s + indexed extraction %.3f s per slab"
@@ -566,7 +586,9 @@ def bench_served(n_req):
     out["plain_saturation"] = {"requests_per_s": round(plain_rps, 1),
                                "gvoxels_per_s": round(pts / pwall0 / 1e9, 3)}
     out["throughput_vs_plain"] = round(sat_rps / plain_rps, 2)
-    for frac in (0.25, 0.5, 0.8):
+    # the two lowest rates lie below the plain path's saturation, where its
+    # latency can be measured beside the server's
+    for frac in (0.05, 0.1, 0.25, 0.5, 0.8):
         rate = frac * sat_rps
         arr = CF.poisson_arrivals(n_req, rate)
         lat, wall = srv.run_trace(reqs, arr)

@@ -297,6 +297,13 @@ SIGNATURES = {
     "gpucad_fft_fill": ([_vp, _vp, _i32, _i32, _i32], C.c_int),
     "gpucad_fft_scale": ([_vp, _f, _u64], C.c_int),
     "gpucad_unit_spectrum": ([_vp, _i32, _i32, _i32, _i32, _vp], C.c_int),
+    "gpucad_fft_plan3d": ([_i32, _i32, _i32], C.c_int),
+    "gpucad_fft_destroy": ([], C.c_int),
+    "gpucad_fft_r2c": ([_vp], C.c_int),
+    "gpucad_fft_c2r": ([_vp], C.c_int),
+    "gpucad_fft_c2c_inverse": ([_vp], C.c_int),
+    "gpucad_add_fft_constant": ([_vp, Float2, _i32], C.c_int),
+    "gpucad_grating_sum": ([_vp, _vp, _i32, _i32, _i32], C.c_int),
     "gpucad_svl_field": ([C.POINTER(SvlParams), _vp, _vp, _vp, C.POINTER(SvlResult), _vp],
                          C.c_int),
     "gpucad_import_external_buffer": ([_i32, _u64, _u64, C.POINTER(_vp), C.POINTER(_vp)],
@@ -561,6 +568,14 @@ class Server:
                          self.device).cpu().numpy().reshape(-1, 3)
         t = _device_view(r.triangles, 3 * r.n_triangles, torch.int32,
                          self.device).cpu().numpy().reshape(-1, 3)
+        return f, v, t
+
+    def result_tensors(self, r, params):
+        """Device views (field, vertices (V, 3), triangles (T, 3)) of a finished
+        request, valid until the request is released."""
+        f = _device_view(r.field, params.npoints(), torch.float32, self.device)
+        v = _device_view(r.vertices, 3 * r.n_vertices, torch.float32, self.device).view(-1, 3)
+        t = _device_view(r.triangles, 3 * r.n_triangles, torch.int32, self.device).view(-1, 3)
         return f, v, t
 
     def run_trace(self, requests, arrivals):

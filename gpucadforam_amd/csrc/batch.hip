@@ -9,15 +9,17 @@
 // batch's mesh buffers.  A run has no memset node, so it can be captured once
 // as a HIP graph and replayed.
 //
-// Brick shape (BX, lz) and waves per CU of the field launch, and the
-// workgroups per CU of the extraction launches, are picked by timing
-// candidates on the batch itself; the choice is cached per device and batch
-// signature (in memory, and in the file GPUCAD_TUNE_CACHE names, if set).
+// The field's brick shape (BX, lz) is picked by timing candidates on the
+// batch itself; the choice is cached per device and batch signature (in
+// memory, and in the file GPUCAD_TUNE_CACHE names, if set: lines
+// "<format tag> <signature> <bx> <lz>", other tags and invalid shapes are
+// skipped on load).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -203,6 +205,14 @@ std::mutex g_tune_mu;
 std::map<std::string, Shape> g_tune;
 bool g_tune_loaded = false;
 
+// Tag of the cache file's lines: a file written by another build of the
+// tuner (other shapes or kernels) is ignored rather than trusted.
+constexpr const char *kTuneFormat = "gpucad-tune-2";
+
+bool valid_shape(const Shape &s) {
+    return (s.bx == 64 || s.bx == 32 || s.bx == 16 || s.bx == 8) && s.lz >= 1 && s.lz <= kMaxLZ;
+}
+
 void tune_load_locked() {
     if (g_tune_loaded) return;
     g_tune_loaded = true;
@@ -210,10 +220,13 @@ void tune_load_locked() {
     if (!path || !*path) return;
     FILE *f = std::fopen(path, "r");
     if (!f) return;
-    char key[256];
-    Shape s;
-    while (std::fscanf(f, "%255s %d %d", key, &s.bx, &s.lz) == 3)
+    char line[512], tag[64], key[256];
+    while (std::fgets(line, sizeof(line), f)) {
+        Shape s;
+        if (std::sscanf(line, "%63s %255s %d %d", tag, key, &s.bx, &s.lz) != 4) continue;
+        if (std::strcmp(tag, kTuneFormat) != 0 || !valid_shape(s)) continue;
         g_tune[key] = s;
+    }
     std::fclose(f);
 }
 
@@ -234,7 +247,7 @@ void tune_store(const std::string &key, const Shape &s) {
     if (!path || !*path) return;
     FILE *f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d\n", key.c_str(), s.bx, s.lz);
+    std::fprintf(f, "%s %s %d %d\n", kTuneFormat, key.c_str(), s.bx, s.lz);
     std::fclose(f);
 }
 
@@ -387,8 +400,8 @@ int batch_create(const gc_lattice_params *grids, int n, const gc_batch_options *
                o->brick_x == 8,
            "brick_x must be 0, 64, 32, 16 or 8");
     GC_ARG(o->brick_lz >= 0 && o->brick_lz <= kMaxLZ, "brick_lz must be 0..%d", kMaxLZ);
-    GC_ARG(o->waves_per_cu >= 0 && o->waves_per_cu <= 32 && o->waves_per_cu % 4 == 0,
-           "waves_per_cu must be 0 or 4..32 in steps of 4");
+    GC_ARG(o->waves_per_cu == 0,
+           "waves_per_cu must be 0: the field runs one workgroup per brick");
     for (int i = 0; i < n; ++i) {
         int st = validate_batch_grid(grids[i], i);
         if (st) return st;

@@ -538,8 +538,8 @@ int gpucad_server_create(const gc_server_options *o, gc_server **out) {
                o->brick_x == 8,
            "brick_x must be 0, 64, 32, 16 or 8");
     GC_ARG(o->brick_lz >= 0 && o->brick_lz <= gc::kMaxLZ, "brick_lz out of range");
-    GC_ARG(o->waves_per_cu >= 0 && o->waves_per_cu <= 32 && o->waves_per_cu % 4 == 0,
-           "waves_per_cu must be 0 or 4..32 in steps of 4");
+    GC_ARG(o->waves_per_cu == 0,
+           "waves_per_cu must be 0: the field runs one workgroup per brick");
     const gc::DeviceTables *tabs = nullptr;
     int st = gc::device_tables(&tabs);
     if (st) return st;

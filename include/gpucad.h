@@ -186,9 +186,9 @@ typedef struct gc_batch_options {
     float iso;                 /* isovalue of every grid */
     int32_t brick_x;           /* points along x per wave row: 64, 32, 16, 8; 0 = autotuned */
     int32_t brick_lz;          /* z planes per brick, 1..128; 0 = autotuned */
-    int32_t waves_per_cu;      /* accepted for compatibility (0 or 4..32 in steps
-                                  of 4) and ignored: the field runs one
-                                  workgroup per brick; reported as 0 */
+    int32_t waves_per_cu;      /* must be 0 (GPUCAD_E_ARG otherwise): the field
+                                  runs one workgroup per brick, so there is no
+                                  resident-wave count to choose; reported as 0 */
     int32_t use_graph;         /* nonzero: a run is one replay of a captured graph */
     int32_t unit_domain;       /* vertex positions: 0 = point indices, 1 = [0,1]^3
                                   (spacing 1/(n-1) per axis) */
@@ -256,7 +256,7 @@ typedef struct gc_server_options {
     float iso;
     int32_t brick_x;           /* field brick (as gc_batch_options); 0 = 64 */
     int32_t brick_lz;          /* 0 = 16 */
-    int32_t waves_per_cu;      /* ignored (see gc_batch_options) */
+    int32_t waves_per_cu;      /* must be 0 (see gc_batch_options) */
 } gc_server_options;
 
 typedef struct gc_request_result {
@@ -432,6 +432,35 @@ int gpucad_fft_fill(const gc_float2 *d_half, gc_float2 *d_full, int32_t nx, int3
                     int32_t nz);
 /* Fft_lattice::fft_scalar (Fft_lattice.h:20, Fft_lattice.cu:124-147): data /= scalar. */
 int gpucad_fft_scale(gc_float2 *d_data, float scalar, uint64_t n);
+
+/* The unit cell's transforms.  gpucad_fft_plan3d records the shape on the
+ * current device, replacing the caller's three cufftPlan3d(&plan, Nxu, Nyu,
+ * Nzu, CUFFT_R2C / C2R / C2C) (src/main.cu:3522-3526): n0 the slowest and n2
+ * the fastest axis, as cufftPlan3d's (nx, ny, nz); axes up to 2048 points.
+ * gpucad_fft_destroy replaces the cufftDestroy calls (src/main.cu:5070-5074).
+ * The transforms are cuFFT's (unnormalised, forward e^{-2 pi i jk/n}, inverse
+ * e^{+2 pi i jk/n}), in place, synchronous; the real layout pads rows of n2
+ * reals to 2 (n2/2 + 1) floats and the half spectrum keeps n2/2 + 1 complex
+ * values per row (direct DFT per axis, FP64 accumulation). */
+int gpucad_fft_plan3d(int32_t n0, int32_t n1, int32_t n2);
+int gpucad_fft_destroy(void);
+/* Fft_lattice::fft_func (Fft_lattice.h:16, Fft_lattice.cu:107-112):
+ * cufftExecR2C in place. */
+int gpucad_fft_r2c(gc_float2 *data);
+/* Fft_lattice::ifft_func (Fft_lattice.h:17, Fft_lattice.cu:114-119):
+ * cufftExecC2R in place (the imaginary parts of each row's first coefficient,
+ * and of its n2/2-th for even n2, are not used). */
+int gpucad_fft_c2r(gc_float2 *data);
+/* Fft_lattice::ifft_func_complex (Fft_lattice.h:18, Fft_lattice.cu:121-126):
+ * cufftExecC2C(..., CUFFT_INVERSE) in place. */
+int gpucad_fft_c2c_inverse(gc_float2 *data);
+/* Fft_lattice::add_fft_constants (Fft_lattice.h:22, Fft_lattice.cu:277-289):
+ * grating[index] = value. */
+int gpucad_add_fft_constant(gc_float2 *grating, gc_float2 value, int32_t index);
+/* Fft_lattice::indiviual_grating_sum (Fft_lattice.h:21, Fft_lattice.cu:255-274):
+ * fft_function[i] += sum_gratings[i].x for i < nx ny nz. */
+int gpucad_grating_sum(const gc_float2 *sum_gratings, float *fft_function, int32_t nx, int32_t ny,
+                       int32_t nz);
 
 /* The (2 range + 1)^3 Fourier coefficients of a unit cell that
  * unit_lattice keeps (src/main.cu:3578-3711: forward real transform scaled

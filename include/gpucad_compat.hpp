@@ -2,13 +2,18 @@
 //
 // Header-only C++ classes with the method signatures of the reference's
 // hot-path classes, so that the reference's caller (class Multitopo,
-// src/main.cu) compiles against them unchanged and runs the MI355X kernels:
+// src/main.cu) makes its hot-path method calls unchanged and runs the MI355X
+// kernels.  What the header does not replace is the caller's own use of the
+// CUDA runtime and libraries around those calls (allocation, copies, events,
+// the Vulkan interop, and the three cufftPlan3d / cufftDestroy calls, which
+// become gpucad_fft_plan3d / gpucad_fft_destroy):
 //
 //   MarchingCubeCuda   src/MarchingCubes_kernel.h:34-124 (path methods)
 //   Isosurface         src/Isosurface.h:10-80 (lattice and boolean extraction)
 //   Modelling          src/Modelling.h:10-48 (primitive fields)
 //   Fft_lattice        src/lattice_files/Fft_lattice.h:10-24 (create_lattice,
-//                      fft_scalar, fft_fill)
+//                      the unit cell's transforms, fft_scalar, fft_fill,
+//                      indiviual_grating_sum, add_fft_constants)
 //   Interpolations     src/Interpolations.h:10-40 (the phase texture of the
 //                      lattice synthesis)
 //   Gratings           src/lattice_files/Gratings.h:10-80 (lattice synthesis,
@@ -379,9 +384,30 @@ public:
             gpucad_create_lattice(d_latticevol, NX, NY, NZ, size, lattice_type_index),
             "create_lattice");
     }
-    // The transforms themselves (fft_func, ifft_func: cuFFT plans owned by
-    // src/main.cu:78-80, 3522-3526) are library calls of the caller;
-    // gpucad_unit_spectrum gives the coefficients the synthesis needs directly.
+    // The transforms run on the shape gpucad_fft_plan3d recorded (the
+    // caller's cufftPlan3d calls, src/main.cu:3522-3526, become one call of
+    // it; cufftDestroy becomes gpucad_fft_destroy).
+    void fft_func(float2 *fft_data) {
+        gpucad_compat::check(gpucad_fft_r2c((gc_float2 *)fft_data), "fft_func");
+    }
+    void ifft_func(float2 *fft_data) {
+        gpucad_compat::check(gpucad_fft_c2r((gc_float2 *)fft_data), "ifft_func");
+    }
+    void ifft_func_complex(float2 *fft_data) {
+        gpucad_compat::check(gpucad_fft_c2c_inverse((gc_float2 *)fft_data), "ifft_func_complex");
+    }
+    void indiviual_grating_sum(float2 *sum_gratings, float *fft_function, int Nx, int Ny,
+                               int Nz) {
+        gpucad_compat::check(
+            gpucad_grating_sum((const gc_float2 *)sum_gratings, fft_function, Nx, Ny, Nz),
+            "indiviual_grating_sum");
+    }
+    void add_fft_constants(float2 *indvidual_grating, float2 fft_constant, int index) {
+        gpucad_compat::check(gpucad_add_fft_constant((gc_float2 *)indvidual_grating,
+                                                     gc_float2{fft_constant.x, fft_constant.y},
+                                                     index),
+                             "add_fft_constants");
+    }
     void fft_scalar(float2 *fft_data_compute, float scalar_val, int size) {
         gpucad_compat::check(gpucad_fft_scale((gc_float2 *)fft_data_compute, scalar_val, size),
                              "fft_scalar");

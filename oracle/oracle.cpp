@@ -370,9 +370,12 @@ void row_edges(const uint8_t *s, int y, int z, int nx, int ny, int nz, RowEdges 
 
 // Canonical indexed extraction (vertex k = k-th crossing edge in id order
 // 3*point+axis; triangle order = cell order, triTable order per cell).
+// z_index0: global z index of the field's first plane (positions use global
+// indices, so a z-slab reproduces the whole grid's vertices bit for bit)
 int extract_indexed(const float *f, int nx, int ny, int nz, float iso, const float *origin,
                     const float *spacing, float *verts, uint64_t vcap, uint32_t *tris,
-                    uint64_t tcap, uint64_t *counts, uint64_t *edge_ids, int threads) {
+                    uint64_t tcap, uint64_t *counts, uint64_t *edge_ids, int threads,
+                    int z_index0) {
     const uint64_t npts = (uint64_t)nx * ny * nz;
     std::vector<uint8_t> s(npts);
 #pragma omp parallel for num_threads(threads) schedule(static)
@@ -409,7 +412,7 @@ int extract_indexed(const float *f, int nx, int ny, int nz, float iso, const flo
             const int y = (int)(r % ny), z = (int)(r / ny);
             row_edges(s.data(), y, z, nx, ny, nz, R[0]);
             // vertices owned by this row
-            const float yf = (float)y, zf = (float)z;
+            const float yf = (float)y, zf = (float)(z + z_index0);
             const float py = fmaf(yf, sy, oy), pz = fmaf(zf, sz, oz);
             for (int x = 0; x < nx; ++x) {
                 const uint8_t c = R[0].cross[x];
@@ -573,6 +576,8 @@ F3 edge_vertex(int mode, F3 p0, F3 p1, float id0, float id1, float a0, float a1,
 void generate(int mode, const float *vol, const float *vol1, const float *vol2, const Grid &g,
               float iso, float lo1, float hi1, float lo2, float hi2, const uint32_t *comp,
               const uint32_t *scan, uint32_t active, uint32_t max_verts, float *pos, float *norm) {
+    // every active voxel writes only its own output range
+#pragma omp parallel for schedule(dynamic, 4096)
     for (uint32_t i = 0; i < active; ++i) {
         const uint32_t voxel = comp[i];
         uint32_t gx, gy, gz;
@@ -704,6 +709,7 @@ void bool_generate(unsigned flags, const GPt *fx, const float *dy, const float *
     // corner's edge parameter); vertexInterp_new walks the plain edge list
     static const int pe[12][3] = {{0, 1, 0}, {1, 2, 1}, {3, 2, 0}, {0, 3, 1}, {4, 5, 0}, {5, 6, 1},
                                   {7, 6, 0}, {4, 7, 1}, {0, 4, 2}, {1, 5, 2}, {2, 6, 2}, {3, 7, 2}};
+#pragma omp parallel for schedule(dynamic, 4096)
     for (uint32_t i = 0; i < active; ++i) {
         const uint32_t voxel = comp[i];
         uint32_t gx, gy, gz;
@@ -847,9 +853,10 @@ void orc_signs(const float *f, int nx, int ny, int nz, float iso, uint64_t *out)
 // ---- canonical indexed extraction ------------------------------------------
 int orc_extract_indexed(const float *f, int nx, int ny, int nz, float iso, const float *origin,
                         const float *spacing, float *verts, uint64_t vcap, uint32_t *tris,
-                        uint64_t tcap, uint64_t *counts, uint64_t *edge_ids, int threads) {
+                        uint64_t tcap, uint64_t *counts, uint64_t *edge_ids, int threads,
+                        int z_index0) {
     return extract_indexed(f, nx, ny, nz, iso, origin, spacing, verts, vcap, tris, tcap, counts,
-                           edge_ids, threads);
+                           edge_ids, threads, z_index0);
 }
 
 // ---- reference-shaped classify / scan / compact / generate -----------------
@@ -858,6 +865,7 @@ int orc_extract_indexed(const float *f, int nx, int ny, int nz, float iso, const
 void orc_classify(const float *vol, int nx, int ny, int nz, float iso, uint8_t *cube,
                   uint32_t *verts, uint32_t *occ) {
     const int cx = nx - 1, cy = ny - 1, cz = nz - 1;
+#pragma omp parallel for schedule(static)
     for (int z = 0; z < cz; ++z)
         for (int y = 0; y < cy; ++y)
             for (int x = 0; x < cx; ++x) {
@@ -881,6 +889,7 @@ void orc_exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n) {
 
 // compactVoxels (src/MarchingCubes_kernel.cu:1594-1608)
 void orc_compact(const uint32_t *occ, const uint32_t *occ_scan, uint64_t n, uint32_t *comp) {
+#pragma omp parallel for schedule(static)
     for (uint64_t i = 0; i < n; ++i)
         if (occ[i]) comp[occ_scan[i]] = (uint32_t)i;
 }
@@ -929,6 +938,7 @@ void orc_bool_classify(unsigned flags, const void *fixed, const float *dyn, cons
     Grid g{(uint32_t)nx, (uint32_t)ny, (uint32_t)nz, (uint32_t)((nx - 1) * (ny - 1)),
            (uint32_t)(nx - 1), 1, 1, 1, 0, 0, 0};
     const uint64_t n = (uint64_t)(nx - 1) * (ny - 1) * (nz - 1);
+#pragma omp parallel for schedule(static)
     for (uint64_t i = 0; i < n; ++i) {
         uint32_t x, y, z;
         grid_pos((uint32_t)i, g, x, y, z);
@@ -944,7 +954,9 @@ void orc_bool_copy(unsigned flags, void *vol_one, const float *vol_two, const fl
     GPt *vo = (GPt *)vol_one;
     const uint64_t n = (uint64_t)nx * ny * nz;
     const bool dynamic = flags & B_DYNAMIC;
-    for (uint64_t i = 0; i + 1 < n; ++i) {
+    // each point rewrites only its own record and reads neighbours' fields
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < n - 1; ++i) {  // n >= 1
         const uint32_t plane = (uint32_t)(nx * ny);
         const uint32_t z = (uint32_t)(i / plane), rem = (uint32_t)(i % plane);
         const uint32_t y = rem / nx, x = rem % nx;

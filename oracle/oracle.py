@@ -39,7 +39,7 @@ def _load():
         "orc_lattice_field_f64": ([vp, f64p, C.c_int, C.c_int, C.c_int], None),
         "orc_signs": ([f32p, C.c_int, C.c_int, C.c_int, C.c_float, u64p], None),
         "orc_extract_indexed": ([f32p, C.c_int, C.c_int, C.c_int, C.c_float, f32p, f32p, vp,
-                                 C.c_uint64, vp, C.c_uint64, u64p, vp, C.c_int], C.c_int),
+                                 C.c_uint64, vp, C.c_uint64, u64p, vp, C.c_int, C.c_int], C.c_int),
         "orc_classify": ([f32p, C.c_int, C.c_int, C.c_int, C.c_float, vp, vp, vp], None),
         "orc_exclusive_scan_u32": ([u32p, u32p, C.c_uint64], None),
         "orc_compact": ([u32p, u32p, C.c_uint64, u32p], None),
@@ -151,8 +151,10 @@ def signs(field, nx, ny, nz, iso=0.0):
 
 
 def extract_indexed(field, nx, ny, nz, iso=0.0, origin=(0, 0, 0), spacing=None, vcap=None,
-                    tcap=None, edge_ids=False, nthreads=None):
-    """Canonical indexed mesh: (vertices (V,3) f32, triangles (T,3) u32, edge ids or None)."""
+                    tcap=None, edge_ids=False, nthreads=None, z_index0=0):
+    """Canonical indexed mesh: (vertices (V,3) f32, triangles (T,3) u32, edge ids or None).
+    z_index0: global z index of the field's first plane (vertex positions of a
+    z-slab then equal those of the whole grid's mesh)."""
     field = np.ascontiguousarray(field, np.float32)
     if spacing is None:
         spacing = (1.0 / (nx - 1), 1.0 / (ny - 1), 1.0 / (nz - 1))
@@ -161,16 +163,28 @@ def extract_indexed(field, nx, ny, nz, iso=0.0, origin=(0, 0, 0), spacing=None, 
     s = np.asarray(spacing, np.float32)
     if vcap is None:
         lib.orc_extract_indexed(field, nx, ny, nz, iso, o, s, None, 0, None, 0, counts, None,
-                                threads(nthreads))
+                                threads(nthreads), z_index0)
         vcap, tcap = int(counts[0]), int(counts[1])
     verts = np.zeros((max(vcap, 1), 3), np.float32)
     tris = np.zeros((max(tcap, 1), 3), np.uint32)
     eids = np.zeros(max(vcap, 1), np.uint64) if edge_ids else None
     lib.orc_extract_indexed(field, nx, ny, nz, iso, o, s, verts.ctypes.data, vcap,
                             tris.ctypes.data, tcap, counts,
-                            None if eids is None else eids.ctypes.data, threads(nthreads))
+                            None if eids is None else eids.ctypes.data, threads(nthreads),
+                            z_index0)
     nv, nt = int(counts[0]), int(counts[1])
     return verts[:nv], tris[:nt], (eids[:nv] if edge_ids else None)
+
+
+def mesh_counts(field, nx, ny, nz, iso=0.0, nthreads=None):
+    """(vertices, triangles) of the canonical mesh of a field (no output)."""
+    field = np.ascontiguousarray(field, np.float32)
+    counts = np.zeros(2, np.uint64)
+    o = np.zeros(3, np.float32)
+    s = np.ones(3, np.float32)
+    lib.orc_extract_indexed(field, nx, ny, nz, iso, o, s, None, 0, None, 0, counts, None,
+                            threads(nthreads), 0)
+    return int(counts[0]), int(counts[1])
 
 
 def classify(vol, nx, ny, nz, iso):
@@ -402,3 +416,56 @@ def fft_fill(half, nx, ny, nz):
     full = np.zeros(2 * nx * ny * nz, np.float32)
     lib.orc_fft_fill(h, full, nx, ny, nz)
     return full
+
+
+# ---- Fft_lattice's transforms (src/lattice_files/Fft_lattice.cu:107-126) ---
+# cuFFT 12.3 (the reference's dependency, Makefile:51 -lcufft) semantics,
+# restated as DFT matrices applied axis by axis in float64: cufftPlan3d(n0,
+# n1, n2) with n2 the fastest axis; unnormalised; forward e^{-2 pi i jk/n},
+# inverse e^{+2 pi i jk/n}.  In-place real data pads each row of n2 reals to
+# 2 (n2/2 + 1) floats; the half spectrum holds n2/2 + 1 complex values per
+# row.  Complex-to-real: inverse along axes 0 and 1, then along the rows the
+# Hermitian sum Re X0 + 2 sum_k Re(Xk e^{+...}) (+ the Nyquist term for even
+# n2), i.e. the imaginary parts of X0 (and X[n2/2]) are not used.
+
+def _dft_matrix(n, sign):
+    j = np.arange(n)
+    m = np.outer(j, j) % n  # exact reduction of jk before the angle
+    return np.exp(sign * 2j * np.pi * m / n)
+
+
+def _along(a, axis, mat):
+    return np.moveaxis(np.tensordot(mat, a, axes=([1], [axis])), 0, axis)
+
+
+def fft_r2c_padded(buf, n0, n1, n2):
+    """Forward transform of the in-place real layout -> (n0, n1, n2//2+1) complex128."""
+    h = n2 // 2 + 1
+    x = np.asarray(buf, np.float32).reshape(n0, n1, 2 * h)[:, :, :n2].astype(np.float64)
+    X = _along(x, 2, _dft_matrix(n2, -1)[:h])
+    X = _along(X, 1, _dft_matrix(n1, -1))
+    return _along(X, 0, _dft_matrix(n0, -1))
+
+
+def fft_c2r(X, n0, n1, n2):
+    """Unnormalised inverse of a (n0, n1, n2//2+1) half spectrum -> (n0, n1, n2) float64."""
+    h = n2 // 2 + 1
+    Y = _along(np.asarray(X, np.complex128).reshape(n0, n1, h), 0, _dft_matrix(n0, 1))
+    Y = _along(Y, 1, _dft_matrix(n1, 1))
+    j = np.arange(n2)
+    kh = (n2 - 1) // 2
+    k = np.arange(1, kh + 1)
+    ang = 2 * np.pi * (np.outer(j, k) % n2) / n2
+    x = Y[..., :1].real + 2 * (Y[..., 1:kh + 1].real @ np.cos(ang).T -
+                               Y[..., 1:kh + 1].imag @ np.sin(ang).T)
+    if n2 % 2 == 0:
+        x = x + Y[..., n2 // 2:n2 // 2 + 1].real * np.where(j % 2 == 0, 1.0, -1.0)
+    return x
+
+
+def fft_c2c_inverse(X, n0, n1, n2):
+    """Unnormalised inverse complex transform of (n0, n1, n2) values."""
+    Y = np.asarray(X, np.complex128).reshape(n0, n1, n2)
+    for axis, n in ((2, n2), (1, n1), (0, n0)):
+        Y = _along(Y, axis, _dft_matrix(n, 1))
+    return Y

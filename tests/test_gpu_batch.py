@@ -2,9 +2,10 @@
 
 A grid's field, sign words and mesh inside a batch must be bit-identical to
 what the one-grid entry points (gpucad_lattice_field + gpucad_extract)
-produce for it alone, whatever the brick shape, the waves per CU, graph
-replay or not; a couple of grids are also checked against the oracle
-directly.  Graph replays are repeated with eager work in between and every
+produce for it alone, whatever the brick shape, graph replay or not; a
+couple of grids are also checked against the oracle directly, and the
+256-grid batched workload (seed 0xBA7C) runs in one graph replay with every
+grid compared with its alone run.  Graph replays are repeated with eager work in between and every
 replay is compared.
 """
 import numpy as np
@@ -77,7 +78,7 @@ def _check_batch(b, alone, label):
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_matches_alone(gc, dev, alone, bx, graph):
     print("batch bx=%d graph=%s grids=%d" % (bx, graph, len(GRIDS)), flush=True)
-    b = gc.Batch(GRIDS, brick_x=bx, brick_lz=8 if bx < 64 else 16, waves_per_cu=8,
+    b = gc.Batch(GRIDS, brick_x=bx, brick_lz=8 if bx < 64 else 16,
                  use_graph=graph)
     try:
         b.run()
@@ -87,7 +88,7 @@ def test_batch_matches_alone(gc, dev, alone, bx, graph):
 
 
 def test_graph_replays_are_identical(gc, dev, alone):
-    b = gc.Batch(GRIDS, brick_x=16, brick_lz=16, waves_per_cu=16, use_graph=True)
+    b = gc.Batch(GRIDS, brick_x=16, brick_lz=16, use_graph=True)
     try:
         for r in range(3):
             # eager work between replays, on the same stream
@@ -126,7 +127,7 @@ def test_autotuned_batch(gc, dev, alone):
 
 def test_batch_against_oracle(gc, orc, dev):
     grids = [CF.plain_gyroid(40), CF.varying_schwarz_p(33)]
-    b = gc.Batch(grids, brick_x=8, brick_lz=4, waves_per_cu=4, use_graph=True)
+    b = gc.Batch(grids, brick_x=8, brick_lz=4, use_graph=True)
     try:
         b.run()
         res = b.results()
@@ -145,7 +146,7 @@ def test_batch_against_oracle(gc, orc, dev):
 def test_batch_overflow_reported(gc, dev, alone):
     total_v = sum(len(a[1]) for a in alone)
     total_t = sum(len(a[2]) for a in alone)
-    b = gc.Batch(GRIDS, brick_x=64, brick_lz=16, waves_per_cu=8, use_graph=False,
+    b = gc.Batch(GRIDS, brick_x=64, brick_lz=16, use_graph=False,
                  vert_capacity=total_v // 2, tri_capacity=total_t)
     try:
         b.run()
@@ -174,3 +175,45 @@ def test_batch_bad_arguments(gc, dev):
     bad.nx = 1
     with pytest.raises(gc.GpucadError):
         gc.Batch([CF.plain_gyroid(32), bad])
+
+
+def test_batched_workload_256_grids(gc, orc, dev):
+    """The batched workload as specified: 256 grids from seed 0xBA7C (per-axis
+    sizes U{32,...,256}, gyroid / Schwarz-P / diamond sheets) in one replay of
+    the captured graph, autotuned brick; every grid's field, vertices and
+    triangles bit-identical to its alone run (compared on the device), and 8
+    grids against the oracle."""
+    grids = list(CF.batched_grids(256, seed=0xBA7C))
+    b = gc.Batch(grids, use_graph=True)
+    try:
+        print("batched 256: tuned %s" % b.tuning(), flush=True)
+        b.fields.fill_(float("nan"))
+        b.vertices.zero_()
+        b.triangles.fill_(-1)
+        torch.cuda.synchronize()
+        b.run()
+        res = b.results()
+        iv = lambda t: t.contiguous().view(torch.int32)
+        for i, p in enumerate(grids):
+            f = torch.empty(p.npoints(), dtype=torch.float32, device=dev)
+            s = torch.zeros(gc.sign_words(p.nx, p.ny, p.nz), dtype=torch.int64, device=dev)
+            gc.lattice_field(p, out=f, signs=s, iso=0.0)
+            m = gc.extract(f, p.nx, p.ny, p.nz, signs=s)
+            bm = b.mesh(i, res)
+            assert torch.equal(iv(b.field(i, res)), iv(f)), "grid %d field" % i
+            assert (bm.n_vertices, bm.n_triangles) == (m.n_vertices, m.n_triangles), i
+            assert torch.equal(iv(bm.vertices), iv(m.vertices)), "grid %d vertices" % i
+            assert torch.equal(bm.triangles, m.triangles), "grid %d triangles" % i
+        for i in range(0, 256, 32):
+            p = grids[i]
+            print("batched 256: grid %d %dx%dx%d against the oracle" % (i, p.nx, p.ny, p.nz),
+                  flush=True)
+            ref_f = orc.lattice_field_f32(p)
+            assert np.array_equal(to_np(b.field(i, res)).view(np.uint32), ref_f.view(np.uint32))
+            v, t, _ = orc.extract_indexed(ref_f, p.nx, p.ny, p.nz)
+            bm = b.mesh(i, res)
+            assert (bm.n_vertices, bm.n_triangles) == (len(v), len(t))
+            assert np.array_equal(to_np(bm.vertices).view(np.uint32), v.view(np.uint32))
+            assert np.array_equal(to_np(bm.triangles).astype(np.uint32), t)
+    finally:
+        b.close()

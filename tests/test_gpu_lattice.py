@@ -13,7 +13,8 @@ import pytest
 import torch
 
 from gpucadforam_amd import configs as CF
-from tests.helpers import edge_use_counts, euler_characteristic, oriented_consistently, to_np, ulp_diff
+from tests.helpers import (edge_use_counts, euler_characteristic, mesh_slab_against_oracle,
+                           oriented_consistently, to_np, ulp_diff)
 
 pytestmark = pytest.mark.gpu
 
@@ -188,9 +189,34 @@ def test_linear_field_vertices_exact(gc, orc, dev):
     assert np.abs(resid).max() < 2e-6
 
 
+def _slab_io(fv, m):
+    """Host views of planes of a device field and ranges of a device mesh."""
+    planes = lambda a, b: to_np(fv[a:b]).reshape(-1)
+    verts = lambda i, j: to_np(m.vertices[i:j])
+    tris = lambda i, j: to_np(m.triangles[i:j]).astype(np.uint32)
+    return planes, verts, tris
+
+
+def _edges_on_device(t, nv):
+    """Edge-use histogram values and the directed-edge uniqueness of a device
+    triangle list (T, 3): (max uses of an undirected edge, undirected edges
+    used once as (a, b) pairs on the host, whether every directed edge is
+    unique = consistent winding)."""
+    t = t.long()
+    a = torch.cat([t[:, 0], t[:, 1], t[:, 2]])
+    b = torch.cat([t[:, 1], t[:, 2], t[:, 0]])
+    d = torch.sort(a * nv + b).values
+    wound = bool((d[1:] != d[:-1]).all())
+    del d
+    u, cnt = torch.unique(torch.minimum(a, b) * nv + torch.maximum(a, b), return_counts=True)
+    once = u[cnt == 1]
+    return int(cnt.max()), torch.stack([once // nv, once % nv], 1), wound
+
+
 def test_full_size_varying_schwarz_p(gc, orc, dev):
-    """512^3 spatially varying Schwarz-P ∩ sphere: oracle slabs of the field and
-    the size-independent mesh invariants of the whole extraction."""
+    """512^3 spatially varying Schwarz-P ∩ sphere: oracle slabs of the field,
+    oracle slabs of the mesh (bottom, middle, top: triangle ids and order,
+    vertex bits) and the mesh invariants of the whole extraction."""
     p = CF.varying_schwarz_p(512)
     f, s = gpu_field(gc, p)
     fv = f.view(512, 512, 512)
@@ -201,6 +227,15 @@ def test_full_size_varying_schwarz_p(gc, orc, dev):
     t = m.triangles
     assert m.n_triangles > 1_000_000
     assert int(t.min()) >= 0 and int(t.max()) < m.n_vertices
+    sp = (1 / 511, 1 / 511, 1 / 511)
+    planes, verts, tris = _slab_io(fv, m)
+    # the sphere (r = 0.45) spans planes 26..485: its bottom cap, the middle,
+    # and a top slab that reaches the last plane (offsets from the totals)
+    for z0, z1 in ((16, 40), (240, 272), (470, 511)):
+        print("512^3 mesh slab z %d..%d against the oracle" % (z0, z1), flush=True)
+        nt, nv = mesh_slab_against_oracle(orc, planes, 512, 512, 512, sp, z0, z1, verts, tris,
+                                          (m.n_vertices, m.n_triangles))
+        assert nt > 10_000
     tn = to_np(t)
     assert edge_use_counts(tn) == {2: 3 * len(tn) // 2}  # closed (the sphere is interior)
     assert oriented_consistently(tn)
@@ -211,17 +246,40 @@ def test_full_size_varying_schwarz_p(gc, orc, dev):
 
 
 def test_full_size_bent_diamond(gc, orc, dev):
-    """1024^3 bent diamond ∪ box (4 GiB field): oracle slabs and mesh invariants."""
-    p = CF.bent_diamond(1024)
+    """1024^3 bent diamond ∪ box (4 GiB field): oracle slabs of the field and
+    of the mesh (triangle ids and order, vertex bits), and on the device: every
+    vertex used, every edge used once or twice, edges used once only on the
+    domain's boundary faces, consistent winding."""
+    n = 1024
+    p = CF.bent_diamond(n)
     f, s = gpu_field(gc, p)
-    fv = f.view(1024, 1024, 1024)
+    fv = f.view(n, n, n)
     for z0 in (0, 700):
         ref = orc.lattice_field_f32(p, z0, z0 + 2)
         assert ulp_diff(to_np(fv[z0:z0 + 2]).reshape(-1), ref).max() <= FIELD_ULP
-    m = gc.extract(f, 1024, 1024, 1024, signs=s, vert_capacity=80_000_000,
-                   tri_capacity=160_000_000)
+    m = gc.extract(f, n, n, n, signs=s, vert_capacity=300_000_000, tri_capacity=600_000_000)
     assert m.n_triangles > 10_000_000
     t = m.triangles
     assert int(t.min()) >= 0 and int(t.max()) < m.n_vertices
+    sp = (1 / (n - 1),) * 3
+    planes, verts, tris = _slab_io(fv, m)
+    for z0, z1 in ((0, 16), (500, 520), (1010, 1023)):
+        print("1024^3 mesh slab z %d..%d against the oracle" % (z0, z1), flush=True)
+        nt, nv = mesh_slab_against_oracle(orc, planes, n, n, n, sp, z0, z1, verts, tris,
+                                          (m.n_vertices, m.n_triangles))
+        assert nt > 0
     del f, s, fv
     torch.cuda.empty_cache()
+    print("1024^3 edge use and winding (%d triangles)" % m.n_triangles, flush=True)
+    used = torch.zeros(m.n_vertices, dtype=torch.bool, device=dev)
+    used[t.reshape(-1).long()] = True
+    assert bool(used.all())
+    del used
+    max_use, once, wound = _edges_on_device(t, m.n_vertices)
+    assert max_use == 2 and wound
+    # an edge used by one triangle lies in a boundary face of the domain: both
+    # ends share a coordinate equal to 0 or to the last grid plane's
+    hi = float(np.float32(np.float32(n - 1) * np.float32(sp[0])))
+    va, vb = m.vertices[once[:, 0]], m.vertices[once[:, 1]]
+    on = lambda c: ((va[:, c] == 0) & (vb[:, c] == 0)) | ((va[:, c] == hi) & (vb[:, c] == hi))
+    assert len(once) > 0 and bool((on(0) | on(1) | on(2)).all())

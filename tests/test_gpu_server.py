@@ -181,3 +181,47 @@ def test_server_bad_arguments(gc, dev):
             srv.release(12345)
     finally:
         srv.close()
+
+
+def test_served_workload_poisson_matches_alone(gc, dev):
+    """The served workload's own settings: a 64 GB admission budget and
+    requests from seed 0x5E7E (cubic edge lognormal median 96 capped at 384,
+    gyroid / Schwarz-P / diamond sheet with a sphere or box boolean)
+    submitted at their Poisson arrival times, at a rate that keeps several
+    groups in flight.  Every request's field and mesh equal its alone run
+    (one-grid entry points) bit for bit, compared on the device."""
+    n = 300
+    reqs = CF.served_requests(n, seed=0x5E7E)
+    arr = CF.poisson_arrivals(n, 20000.0)
+    srv = gc.Server(64 << 30)
+    try:
+        ticket_req, got, i = {}, {}, 0
+        t0 = time.perf_counter()
+        while len(got) < n:
+            now = time.perf_counter() - t0
+            while i < n and arr[i] <= now:
+                ticket_req[srv.submit(reqs[i])] = i
+                i += 1
+            for r in srv.poll():
+                got[r.ticket] = r
+            assert time.perf_counter() - t0 < 90, "server did not finish"
+        st = srv.stats()
+        print("served workload: %s" % st, flush=True)
+        assert st["completed"] == n and st["mean_group_size"] > 1.0
+        assert st["peak_bytes_in_use"] <= st["budget_bytes"]
+        iv = lambda t: t.contiguous().view(torch.int32)
+        for tk, r in got.items():
+            p = reqs[ticket_req[tk]]
+            assert r.status == 0
+            f, v, t = srv.result_tensors(r, p)
+            fa = torch.empty(p.npoints(), dtype=torch.float32, device=dev)
+            sa = torch.zeros(gc.sign_words(p.nx, p.ny, p.nz), dtype=torch.int64, device=dev)
+            gc.lattice_field(p, out=fa, signs=sa, iso=0.0)
+            m = gc.extract(fa, p.nx, p.ny, p.nz, signs=sa)
+            assert torch.equal(iv(f), iv(fa)), "request %d field" % ticket_req[tk]
+            assert (len(v), len(t)) == (m.n_vertices, m.n_triangles)
+            assert torch.equal(iv(v), iv(m.vertices)), "request %d vertices" % ticket_req[tk]
+            assert torch.equal(t, m.triangles), "request %d triangles" % ticket_req[tk]
+            srv.release(tk)
+    finally:
+        srv.close()

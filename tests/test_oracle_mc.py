@@ -203,3 +203,25 @@ def test_normalise_four_semantics(orc):
     # all-positive input: the seed 0 becomes the minimum
     ind2, nrm2 = orc.normalise_four(np.full(nx * ny * nz, 3.0, np.float32), nx, ny, nz, 0.2, 0.3)
     assert np.all(nrm2.reshape(nz, ny, nx)[1:-1, 1:-1, 1:-1] == 1.0)
+
+
+def test_slab_comparison_reproduces_the_whole_mesh(orc):
+    """The z-slab comparison used on full-size GPU meshes, checked on the
+    oracle's own whole-grid mesh: every slab (bottom, interior, top) matches,
+    and a one-plane shift of the slab's triangles is caught."""
+    from gpucadforam_amd import configs as CF
+    from tests.helpers import mesh_slab_against_oracle
+    p = CF.varying_schwarz_p(48)
+    f = orc.lattice_field_f32(p)
+    sp = (1 / (p.nx - 1), 1 / (p.ny - 1), 1 / (p.nz - 1))
+    v, t, _ = orc.extract_indexed(f, p.nx, p.ny, p.nz, spacing=sp)
+    plane = p.nx * p.ny
+    planes = lambda a, b: f[a * plane:b * plane]
+    for z0, z1 in ((0, 6), (17, 30), (40, 47)):
+        for totals in (None, (len(v), len(t))):
+            nt, nv = mesh_slab_against_oracle(orc, planes, p.nx, p.ny, p.nz, sp, z0, z1,
+                                              lambda i, j: v[i:j], lambda i, j: t[i:j], totals)
+            assert nt > 0 and nv > 0
+    with pytest.raises(AssertionError):
+        mesh_slab_against_oracle(orc, planes, p.nx, p.ny, p.nz, sp, 17, 30,
+                                 lambda i, j: v[i:j], lambda i, j: t[i + 1:j + 1])
