@@ -264,6 +264,9 @@ SIGNATURES = {
     "gpucad_compute_isosurface_lattice": (
         [_vp, _vp, _vp, _f, _u32, _vp, _vp, _vp, _vp, Uint3, Uint3, Uint3, Float3, Float3,
          C.POINTER(_u32), C.POINTER(_u32), _vp, _u32, _vp, _vp, _f, _f, _f, _f], C.c_int),
+    "gpucad_compute_isosurface_lattice_noalloc": (
+        [_vp, _vp, _vp, _f, _u32, _vp, _vp, _vp, _vp, Uint3, Uint3, Uint3, Float3, Float3,
+         C.POINTER(_u32), C.POINTER(_u32), _vp, _u32, _vp, _vp, _f, _f, _f, _f], C.c_int),
     "gpucad_compute_isosurface_latticeone": (
         [_vp, _vp, _vp, _f, _u32, _vp, _vp, _vp, _vp, Uint3, Uint3, Uint3, Float3, Float3,
          C.POINTER(_u32), C.POINTER(_u32), _vp, _u32, _vp, _f, _f], C.c_int),

@@ -183,7 +183,8 @@ int gpucad_init(int device) {
     int st = gc::device_tables(&t);
     if (st) return st;
     uint32_t *hw = nullptr;
-    return gc::host_words(&hw);
+    if ((st = gc::host_words(&hw))) return st;
+    return gc::noalloc_setup();
 }
 
 int gpucad_shutdown(void) {

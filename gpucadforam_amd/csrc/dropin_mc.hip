@@ -13,6 +13,15 @@
 #include "gc_common.hpp"
 #include "wave_ops.hpp"
 
+// Fixed scratch of gpucad_compute_isosurface_lattice_noalloc: the tile sums
+// of its two scans (occupancy, then vertex counts), kNoAllocMaxTiles words
+// each, 2 MiB per device, held until the process ends.  Host code never uses
+// the array's name as an address: noalloc_setup resolves the array's address
+// on each device and the entry point passes that pointer to the kernels.
+constexpr uint32_t kNoAllocMaxTiles = 1u << 18;
+constexpr uint32_t kNoAllocMaxVoxels = 1u << 30;
+static __device__ __align__(256) uint32_t scratch[2 * kNoAllocMaxTiles];
+
 namespace gc {
 namespace {
 
@@ -71,6 +80,8 @@ __global__ void classify_kernel(uint32_t *verts, uint32_t *occ, const float *vol
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
+static_assert((uint64_t)kNoAllocMaxTiles * kScanTile == kNoAllocMaxVoxels,
+              "the fixed scratch holds the tile sums of the largest allocation-free grid");
 
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *total) {
     __shared__ uint32_t s[kScanThreads];
@@ -477,13 +488,14 @@ int compact(uint32_t *comp, const uint32_t *occ, const uint32_t *occ_scan, uint3
 }
 
 template <int MODE>
-int generate(GenArgs &a, hipStream_t st) {
+int generate(GenArgs &a, hipStream_t st, const DeviceTables *t = nullptr) {
     GC_ARG(a.pos && a.norm && a.comp && a.scan && a.vol, "NULL buffer");
     if (MODE != GEN_STD) GC_ARG(a.vol1, "volume_one is NULL");
     if (MODE == GEN_BAND2) GC_ARG(a.vol2, "volume_two is NULL");
-    const DeviceTables *t = nullptr;
-    int r = device_tables(&t);
-    if (r) return r;
+    if (!t) {
+        int r = device_tables(&t);
+        if (r) return r;
+    }
     a.tri = t->tri_table;
     a.nverts = t->num_verts;
     if (a.active == 0) return GPUCAD_OK;
@@ -504,10 +516,11 @@ __global__ void last_pair_kernel(const uint32_t *a, const uint32_t *b, uint32_t 
 // host memory, read after the stream synchronisation (the reference copies
 // each with its own cudaMemcpy, Isosurface.cu:424-436, 453-472)
 int read_last_pair(const uint32_t *a, const uint32_t *b, uint32_t n, uint32_t *h,
-                   hipStream_t st) {
-    uint32_t *hw = nullptr;
-    int r = host_words(&hw);
-    if (r) return r;
+                   hipStream_t st, uint32_t *hw = nullptr) {
+    if (!hw) {
+        int r = host_words(&hw);
+        if (r) return r;
+    }
     hipLaunchKernelGGL(last_pair_kernel, dim3(1), dim3(64), 0, st, a, b, n, hw);
     GC_CHECK_LAUNCH();
     GC_HIP(hipStreamSynchronize(st));
@@ -517,13 +530,16 @@ int read_last_pair(const uint32_t *a, const uint32_t *b, uint32_t n, uint32_t *h
 }
 
 // Isosurface.cu:401-486 / 488-572 orchestration; MODE selects the generate.
+// With `fixed` (2 * kNoAllocMaxTiles words of device memory) the scans'
+// tile sums go there and the tables and host words must already exist:
+// nothing is allocated.
 template <int MODE>
 int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float iso, uint32_t n,
                        uint32_t *d_verts, uint32_t *d_verts_scan, uint32_t *d_occ,
                        uint32_t *d_occ_scan, gc_uint3 s, gc_uint3 sh, gc_uint3 m, gc_float3 vs,
                        gc_float3 c, uint32_t *active, uint32_t *total, uint32_t *d_comp,
                        uint32_t max_verts, const float *vol1, const float *vol2, float lo1,
-                       float hi1, float lo2, float hi2) {
+                       float hi1, float lo2, float hi2, uint32_t *fixed = nullptr) {
     GC_ARG(active && total, "host outputs are NULL");
     GC_ARG(n > 0, "numVoxels must be > 0");
     GC_ARG(d_verts && d_occ && vol && d_occ_scan, "NULL buffer");
@@ -531,12 +547,18 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
     int r = check_grid(s, sh, m, n);
     if (r) return r;
     const DeviceTables *t = nullptr;
-    if ((r = device_tables(&t))) return r;
+    uint32_t *hw = nullptr;
+    if ((r = fixed ? tables_if_ready(&t) : device_tables(&t))) return r;
+    if ((r = fixed ? host_words_if_ready(&hw) : host_words(&hw))) return r;
     // scratch: the tile sums of occ and verts
     const uint32_t nb = (uint32_t)(((uint64_t)n + kScanTile - 1) / kScanTile);
-    void *ws = nullptr;
-    if ((r = scratch_arena(2 * (size_t)nb * sizeof(uint32_t), &ws))) return r;
-    uint32_t *bsum_o = (uint32_t *)ws, *bsum_v = bsum_o + nb;
+    uint32_t *bsum_o = fixed, *bsum_v = fixed + kNoAllocMaxTiles;
+    if (!fixed) {
+        void *ws = nullptr;
+        if ((r = scratch_arena(2 * (size_t)nb * sizeof(uint32_t), &ws))) return r;
+        bsum_o = (uint32_t *)ws;
+        bsum_v = bsum_o + nb;
+    }
     // classify (src/Isosurface.cu:408-420) with the first pass of both scans
     const GridDesc g = grid_desc(s, sh, m, gc_float3{1, 1, 1}, gc_float3{0, 0, 0});
     hipLaunchKernelGGL(classify_tile_kernel, dim3(nb), dim3(kScanThreads), 0, st, d_verts, d_occ,
@@ -546,7 +568,7 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
     // no voxel is active, as the reference's early return leaves it)
     if ((r = scan_finish(d_occ_scan, d_occ, n, bsum_o, nb, st, d_comp))) return r;
     uint32_t last[2] = {0, 0};
-    if ((r = read_last_pair(d_occ, d_occ_scan, n, last, st))) return r;
+    if ((r = read_last_pair(d_occ, d_occ_scan, n, last, st, hw))) return r;
     *active = last[0] + last[1];
     if (*active == 0) {
         *total = 0;
@@ -554,7 +576,7 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
     }
     GC_ARG(d_verts_scan && d_comp, "NULL buffer");
     if ((r = scan_finish(d_verts_scan, d_verts, n, bsum_v, nb, st, nullptr))) return r;
-    if ((r = read_last_pair(d_verts, d_verts_scan, n, last, st))) return r;
+    if ((r = read_last_pair(d_verts, d_verts_scan, n, last, st, hw))) return r;
     *total = last[0] + last[1];
     GenArgs a;
     a.pos = (float4 *)pos;
@@ -572,13 +594,40 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
     a.hi2 = hi2;
     a.active = *active;
     a.max_verts = max_verts;
-    r = generate<MODE>(a, st);
+    r = generate<MODE>(a, st, t);
     if (r) return r;
     GC_HIP(hipStreamSynchronize(st));
     return GPUCAD_OK;
 }
 
+constexpr int kNoAllocDevices = 64;
+uint32_t *g_fixed_scratch[kNoAllocDevices];  // the array's address on each device
+std::mutex g_fixed_mu;
+
+int fixed_scratch(uint32_t **out) {
+    int dev = 0;
+    GC_HIP(hipGetDevice(&dev));
+    GC_ARG(dev >= 0 && dev < kNoAllocDevices, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_fixed_mu);
+    GC_ARG(g_fixed_scratch[dev], "gpucad_init was not called on device %d", dev);
+    *out = g_fixed_scratch[dev];
+    return GPUCAD_OK;
+}
+
 }  // namespace
+
+int noalloc_setup() {
+    int dev = 0;
+    GC_HIP(hipGetDevice(&dev));
+    GC_ARG(dev >= 0 && dev < kNoAllocDevices, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_fixed_mu);
+    if (!g_fixed_scratch[dev]) {
+        void *p = nullptr;
+        GC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+        g_fixed_scratch[dev] = (uint32_t *)p;
+    }
+    return GPUCAD_OK;
+}
 
 int check_dropin_grid(gc_uint3 s, gc_uint3 sh, gc_uint3 m, uint32_t n) {
     return check_grid(s, sh, m, n);
@@ -733,6 +782,28 @@ int gpucad_compute_isosurface_lattice(const float *vol, gc_float4 *pos, gc_float
         d_voxel_occupied_scan, grid_size, grid_size_shift, grid_size_mask, voxel_size, grid_center,
         active_voxels, total_verts, d_comp_voxel_array, max_verts, vol_one, vol_two, isovalue1,
         isovalue2, iso1, iso2);
+}
+
+int gpucad_compute_isosurface_lattice_noalloc(
+    const float *vol, gc_float4 *pos, gc_float4 *norm, float iso, uint32_t num_voxels,
+    uint32_t *d_voxel_verts, uint32_t *d_voxel_verts_scan, uint32_t *d_voxel_occupied,
+    uint32_t *d_voxel_occupied_scan, gc_uint3 grid_size, gc_uint3 grid_size_shift,
+    gc_uint3 grid_size_mask, gc_float3 voxel_size, gc_float3 grid_center, uint32_t *active_voxels,
+    uint32_t *total_verts, uint32_t *d_comp_voxel_array, uint32_t max_verts, const float *vol_one,
+    const float *vol_two, float isovalue1, float isovalue2, float iso1, float iso2) {
+    clear_error();
+    auto calls = lock_device_calls();  // one fixed scratch per device
+    GC_ARG(num_voxels <= kNoAllocMaxVoxels,
+           "numVoxels %u exceeds the allocation-free limit of %u voxels", num_voxels,
+           kNoAllocMaxVoxels);
+    uint32_t *fixed = nullptr;
+    int r = fixed_scratch(&fixed);
+    if (r) return r;
+    return compute_isosurface<GEN_BAND2>(
+        vol, pos, norm, iso, num_voxels, d_voxel_verts, d_voxel_verts_scan, d_voxel_occupied,
+        d_voxel_occupied_scan, grid_size, grid_size_shift, grid_size_mask, voxel_size, grid_center,
+        active_voxels, total_verts, d_comp_voxel_array, max_verts, vol_one, vol_two, isovalue1,
+        isovalue2, iso1, iso2, fixed);
 }
 
 int gpucad_compute_isosurface_latticeone(const float *vol, gc_float4 *pos, gc_float4 *norm,

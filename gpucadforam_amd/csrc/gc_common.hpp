@@ -84,4 +84,8 @@ int host_words_if_ready(uint32_t **out);
 // each other's pieces).
 std::unique_lock<std::recursive_mutex> lock_device_calls();
 
+// Called by gpucad_init with the device current: resolves the address of the
+// allocation-free drop-in's fixed scratch array on that device.
+int noalloc_setup();
+
 }  // namespace gc

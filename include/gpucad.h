@@ -570,6 +570,22 @@ int gpucad_compute_isosurface_lattice(const float *vol, gc_float4 *pos, gc_float
                                       uint32_t *d_comp_voxel_array, uint32_t max_verts,
                                       const float *vol_one, const float *vol_two,
                                       float isovalue1, float isovalue2, float iso1, float iso2);
+/* gpucad_compute_isosurface_lattice_noalloc: no counterpart in the reference.
+ * For callers that cannot allocate during the call (code around a HIP graph
+ * capture, tight render or serving loops, harnesses that forbid hipMalloc):
+ * the arguments, outputs, status codes and results of
+ * gpucad_compute_isosurface_lattice, with the scans' tile sums in a fixed
+ * per-device array (2 MiB, held until the process ends) and no allocation of
+ * device or host memory.  gpucad_init must have been called on the device
+ * (GPUCAD_E_ARG otherwise); num_voxels <= 2^30 (GPUCAD_E_ARG beyond).  Calls
+ * on one device share the array and are serialised by a per-device lock. */
+int gpucad_compute_isosurface_lattice_noalloc(
+    const float *vol, gc_float4 *pos, gc_float4 *norm, float iso, uint32_t num_voxels,
+    uint32_t *d_voxel_verts, uint32_t *d_voxel_verts_scan, uint32_t *d_voxel_occupied,
+    uint32_t *d_voxel_occupied_scan, gc_uint3 grid_size, gc_uint3 grid_size_shift,
+    gc_uint3 grid_size_mask, gc_float3 voxel_size, gc_float3 grid_center, uint32_t *active_voxels,
+    uint32_t *total_verts, uint32_t *d_comp_voxel_array, uint32_t max_verts, const float *vol_one,
+    const float *vol_two, float isovalue1, float isovalue2, float iso1, float iso2);
 /* Isosurface::computeIsosurface_latticeone (src/Isosurface.h:66-70,
  * src/Isosurface.cu:488-572) */
 int gpucad_compute_isosurface_latticeone(const float *vol, gc_float4 *pos, gc_float4 *norm,
