@@ -20,30 +20,34 @@
 //   signs : ballot of f < iso per 64 points; skipped when the field kernel
 //           already produced the words
 //   count : blocks of 256 words: triangles per wave (case of each active
-//           cell), vertex and triangle sums per block
+//           cell), vertex and triangle sums per block, and each half word's
+//           first vertex number within its block
 //   scan  : one workgroup scans the block sums (64-bit), writes the totals
 //           and the vertex/triangle range of each grid
-//   vert  : block scan of the word vertex counts -> per-word first vertex
-//           number (kept for `tri`); the block's crossing edges are compacted
-//           into an LDS list in canonical order and emitted one lane per edge
-//           (loads of f0 and f1, interpolation); the block's vertices form one
-//           contiguous range, stored coalesced
-//   tri   : one lane per active cell computes the 12 edge vertex numbers from
+//   emit  : the vertex and triangle stages in one launch, their workgroups
+//           interleaved (both depend only on count and scan):
+//     vert: each wave compacts its 64 words' crossing edges into an LDS list
+//           in canonical order and emits them one lane per edge (loads of f0
+//           and f1, interpolation); a wave's vertices form one contiguous
+//           range, stored coalesced
+//     tri : one lane per active cell computes the 12 edge vertex numbers from
 //           popcounts of the crossing words of the four rows touching the cell
-//           and writes its triangles; a wave's triangles form one contiguous
-//           range starting at the wave's scanned base
+//           (plus the rows' first vertex numbers) and writes its triangles; a
+//           wave's triangles form one contiguous range starting at the wave's
+//           scanned base
 //
 // The kernels see grids as segments: a grid's words occupy a contiguous range
-// of global blocks.  Every launch runs one workgroup per block (two per block
-// for the triangle stage); the many-grid form (batched and served) finds each
-// block's grid in a block -> grid table.  Both run the same block code, so a
-// grid's mesh does not depend on what it is launched with.
+// of global blocks.  Every launch runs one workgroup per block (three per
+// block for emit: its vertices and its two triangle halves); the many-grid
+// form (batched and served) finds each block's grid in a block -> grid table.
+// Both run the same block code, so a grid's mesh does not depend on what it
+// is launched with.
 #include "extract_types.hpp"
 
 namespace gc {
 namespace {
 
-constexpr int kVertCap = 4096; // per-block LDS edge-list entries per round
+constexpr int kWaveVertCap = 1024; // per-wave LDS edge-list entries per round
 constexpr int kEdgeStep = 4;   // edges per thread and step of the vertex emission
 
 struct ExtractLayout {
@@ -259,6 +263,7 @@ struct CountArgs {
     const uint64_t *signs;  // sign buffer base
     uint32_t *twave;
     uint64_t *bsum;
+    uint32_t *base;  // block-local first vertex of each half word
 };
 
 template <class Src>
@@ -271,7 +276,7 @@ __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a) {
         const Grid &g = s.g;
         const uint64_t *signs = a.signs + s.sign_off;
         const uint64_t k = (uint64_t)(gb - s.block_off) * kBlock + tid;
-        uint32_t vc = 0, tc = 0;
+        uint32_t vc = 0, tc = 0, vlo = 0;
         if (k < g.words) {
             const WordPos p = word_pos(k, g);
             const bool has_y = p.y + 1 < g.ny, has_z = p.z + 1 < g.nz;
@@ -296,13 +301,16 @@ __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a) {
             const uint64_t ey = has_y ? (wa ^ wb) & vpt : 0ull;
             const uint64_t ez = has_z ? (wa ^ wc) & vpt : 0ull;
             vc = popc64(ex) + popc64(ey) + popc64(ez);
+            vlo = popc64(ex & 0xffffffffull) + popc64(ey & 0xffffffffull) +
+                  popc64(ez & 0xffffffffull);
             if (has_y && has_z) {
                 const uint64_t sd = shift1(wd, dn);
                 const uint64_t act = active_cells(wa, sa, wb, sb, wc, sc, wd, sd, ved);
                 tc = word_triangles(wa, sa, sb, wb, wc, sc, sd, wd, act);
             }
         }
-        const uint32_t vw = readlane32(wave_scan_incl(vc), 63);
+        const uint32_t vincl = wave_scan_incl(vc);
+        const uint32_t vw = readlane32(vincl, 63);
         const uint32_t tincl = wave_scan_incl(tc);
         const uint32_t tlo = readlane32(tincl, 31), tw = readlane32(tincl, 63);
         if (lane == 0) {
@@ -313,6 +321,12 @@ __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a) {
             a.twave[kGroups * (uint64_t)gb + 2 * wave + 1] = tw - tlo;
         }
         __syncthreads();
+        if (k < g.words) {
+            // the word's first vertex within the block, for each 32-point half
+            uint32_t v0 = vincl - vc;
+            for (uint32_t i = 0; i < 4; ++i) v0 += i < wave ? s_sum[i][0] : 0u;
+            *(uint2 *)(a.base + 2 * ((uint64_t)gb * kBlock + tid)) = make_uint2(v0, v0 + vlo);
+        }
         if (tid == 0) {
             a.bsum[2 * (uint64_t)gb] =
                 (uint64_t)s_sum[0][0] + s_sum[1][0] + s_sum[2][0] + s_sum[3][0];
@@ -450,21 +464,28 @@ struct VertArgs {
     const uint64_t *signs;  // sign buffer base
     const uint64_t *bsum;   // scanned block bases
     const SegResult *res;
-    uint32_t *base;
     float *vertices;
     uint64_t vcap;
 };
 
+// The vertex stage's LDS: one edge list per wave (a wave fills and emits its
+// own 64 words' edges, with no wait on the block's other waves).
+struct VertShared {
+    uint16_t list[kBlock / 64][kWaveVertCap + 64];  // + one discard slot per lane
+    uint64_t p0[kBlock];   // linear index of the word's first point
+    uint32_t xyz[kBlock];  // first x | y << 16
+    uint32_t z[kBlock];
+    uint32_t wsum[4];
+};
+
+// Vertices of count block gb.
 template <class Src>
-__global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a) {
-    __shared__ uint16_t s_list[kVertCap + kBlock];  // + one discard slot per thread
-    __shared__ uint64_t s_p0[kBlock];   // linear index of the word's first point
-    __shared__ uint32_t s_xyz[kBlock];  // first x | y << 16
-    __shared__ uint32_t s_z[kBlock];
-    __shared__ uint32_t s_wsum[4];
+__device__ __forceinline__ void vert_block(const Src &src, const VertArgs &a, uint32_t gb,
+                                           VertShared &sh) {
+    uint64_t *s_p0 = sh.p0;
+    uint32_t *s_xyz = sh.xyz, *s_z = sh.z, *s_wsum = sh.wsum;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     {
-        const uint32_t gb = blockIdx.x;
         const uint32_t si = src.seg_of(gb);
         const Seg s = src.get(si);
         const Grid &g = s.g;
@@ -472,7 +493,6 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a) {
         const uint64_t k = (uint64_t)(gb - s.block_off) * kBlock + tid;
         // block base and grid range first: their loads overlap the sign loads
         const uint64_t vblock = a.bsum[2 * (uint64_t)gb];
-        const uint64_t v_off = a.res[si].v_off;
         const uint32_t ovf = a.res[si].ovf;
         uint64_t ex = 0, ey = 0, ez = 0;
         if (k < g.words) {
@@ -493,31 +513,25 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a) {
         }
         const uint32_t vc = popc64(ex) + popc64(ey) + popc64(ez);
         const uint32_t incl = wave_scan_incl(vc);
+        const uint32_t wtot = readlane32(incl, 63);
         if (lane == 63) s_wsum[wave] = incl;
         __syncthreads();
-        uint32_t wbase = 0, vtot = 0;
-        for (uint32_t i = 0; i < 4; ++i) {
+        uint32_t wbase = 0;
+        for (uint32_t i = 0; i < 4; ++i)
             if (i < wave) wbase += s_wsum[i];
-            vtot += s_wsum[i];
-        }
-        const uint32_t vex = wbase + incl - vc;  // block-local first vertex of this word
-        if (k < g.words) {
-            // first vertex numbers of the word's two 32-point halves
-            const uint32_t v0 = (uint32_t)(vblock - v_off + vex);
-            const uint32_t lo = popc64(ex & 0xffffffffull) + popc64(ey & 0xffffffffull) +
-                                popc64(ez & 0xffffffffull);
-            *(uint2 *)(a.base + 2 * ((uint64_t)gb * kBlock + tid)) = make_uint2(v0, v0 + lo);
-        }
+        const uint32_t vexw = incl - vc;  // first vertex of this word within the wave
         if (ovf) return;  // vertex numbers do not fit 32 bits: nothing is written
         const float *field = a.field + s.field_off;
         const uint64_t plane = (uint64_t)g.nx * g.ny;
-        for (uint32_t r0 = 0; r0 < vtot; r0 += kVertCap) {
+        uint16_t *list = sh.list[wave];
+        const uint64_t vwave = vblock + wbase;  // the wave's first vertex
+        for (uint32_t r0 = 0; r0 < wtot; r0 += kWaveVertCap) {
             // this lane's edges, canonical order, that fall into the round's window
-            if (vtot <= kVertCap) {
-                // one round holds the block: no window tests, 32-bit halves;
-                // an edge a point lacks goes to the thread's discard slot
-                uint16_t *dump = s_list + kVertCap + tid;
-                uint32_t i = vex;
+            if (wtot <= kWaveVertCap) {
+                // one round holds the wave: no window tests, 32-bit halves;
+                // an edge a point lacks goes to the lane's discard slot
+                uint16_t *dump = list + kWaveVertCap + lane;
+                uint32_t i = vexw;
 #pragma unroll
                 for (uint32_t h = 0; h < 2; ++h) {
                     const uint32_t hx = (uint32_t)(ex >> (32 * h));
@@ -529,17 +543,17 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a) {
                         pts &= pts - 1;
                         const uint32_t ent = (tid << 8) | ((b + 32 * h) << 2);
                         const uint32_t bx = (hx >> b) & 1u, by = (hy >> b) & 1u, bz = (hz >> b) & 1u;
-                        *(bx ? s_list + i : dump) = (uint16_t)ent;
+                        *(bx ? list + i : dump) = (uint16_t)ent;
                         i += bx;
-                        *(by ? s_list + i : dump) = (uint16_t)(ent | 1u);
+                        *(by ? list + i : dump) = (uint16_t)(ent | 1u);
                         i += by;
-                        *(bz ? s_list + i : dump) = (uint16_t)(ent | 2u);
+                        *(bz ? list + i : dump) = (uint16_t)(ent | 2u);
                         i += bz;
                     }
                 }
-            } else if (vc && vex < r0 + kVertCap && vex + vc > r0) {
+            } else if (vc && vexw < r0 + kWaveVertCap && vexw + vc > r0) {
                 uint64_t pts = ex | ey | ez;
-                uint32_t i = vex;
+                uint32_t i = vexw;
                 while (pts) {
                     const uint32_t b = (uint32_t)__builtin_ctzll(pts);
                     pts &= pts - 1;
@@ -548,23 +562,23 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a) {
 #pragma unroll
                     for (uint32_t ax = 0; ax < 3; ++ax) {
                         if (!((m >> ax) & 1)) continue;
-                        if (i >= r0 && i < r0 + kVertCap)
-                            s_list[i - r0] = (uint16_t)((tid << 8) | (b << 2) | ax);
+                        if (i >= r0 && i < r0 + kWaveVertCap)
+                            list[i - r0] = (uint16_t)((tid << 8) | (b << 2) | ax);
                         ++i;
                     }
                 }
             }
-            __syncthreads();
-            const uint32_t n = min((uint32_t)kVertCap, vtot - r0);
-            // kEdgeStep edges per thread and step, so that all their field
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t n = min((uint32_t)kWaveVertCap, wtot - r0);
+            // kEdgeStep edges per lane and step, so that all their field
             // loads are in flight before the first interpolation
-            for (uint32_t e0 = tid; e0 < n; e0 += kEdgeStep * kBlock) {
+            for (uint32_t e0 = lane; e0 < n; e0 += kEdgeStep * 64) {
                 uint32_t ent[kEdgeStep];
                 float f0[kEdgeStep], f1[kEdgeStep];
 #pragma unroll
                 for (int q = 0; q < kEdgeStep; ++q) {
-                    const uint32_t e = e0 + q * kBlock;
-                    ent[q] = s_list[e < n ? e : e0];
+                    const uint32_t e = e0 + q * 64;
+                    ent[q] = list[e < n ? e : e0];
                     const uint32_t j = ent[q] >> 8, b = (ent[q] >> 2) & 63, ax = ent[q] & 3;
                     const uint64_t pt = s_p0[j] + b;
                     const uint64_t st = ax == 0 ? 1ull : (ax == 1 ? (uint64_t)g.nx : plane);
@@ -573,16 +587,16 @@ __global__ void __launch_bounds__(kBlock) vert_kernel(Src src, VertArgs a) {
                 }
 #pragma unroll
                 for (int q = 0; q < kEdgeStep; ++q) {
-                    const uint32_t e = e0 + q * kBlock;
+                    const uint32_t e = e0 + q * 64;
                     const uint32_t j = ent[q] >> 8, b = (ent[q] >> 2) & 63, ax = ent[q] & 3;
                     const uint32_t xy = s_xyz[j];
                     const V3 v = edge_vertex(s, (float)((xy & 0xffff) + b), (float)(xy >> 16),
                                              (float)s_z[j], ax, f0[q], f1[q]);
-                    const uint64_t id = vblock + r0 + e;
+                    const uint64_t id = vwave + r0 + e;
                     if (e < n && id < a.vcap) *(V3 *)(a.vertices + 3 * id) = v;
                 }
             }
-            __syncthreads();
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
@@ -662,7 +676,7 @@ __device__ __forceinline__ void fill_cells(uint16_t *list, uint32_t act, uint32_
 
 struct TriArgs {
     const uint64_t *signs;  // sign buffer base
-    const uint32_t *base;   // first vertex number of each half word
+    const uint32_t *base;   // block-local first vertex number of each half word
     const uint32_t *twave;  // triangles of each group of 32 words
     const uint64_t *bsum;
     const uint64_t *tri_packed;
@@ -671,25 +685,32 @@ struct TriArgs {
     uint64_t tcap;
 };
 
+// The triangle stage's LDS.
+struct TriShared {
+    static constexpr int kWaves = kTriThreads / 64;
+    uint64_t tri[256];
+    uint16_t list[kWaves][kCellCap];
+    uint32_t vid[kWaves][12 * 64];  // id k of lane l at k*64 + (l ^ k)
+    uint64_t pk[kWaves][64];        // packed triangle list of each lane's cell
+    uint16_t own[kWaves][5 * 64];   // triangle -> lane | (triangle of the cell) << 8
+};
+
+// Triangles of triangle block tb (half tb & 1 of count block tb >> 1).
 template <class Src>
-__global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a) {
-    constexpr int kWaves = kTriThreads / 64;
-    __shared__ uint64_t s_tri[256];
-    __shared__ uint16_t s_list[kWaves][kCellCap];
-    __shared__ uint32_t s_vid[kWaves][12 * 64];  // id k of lane l at k*64 + (l ^ k)
-    __shared__ uint64_t s_pk[kWaves][64];       // packed triangle list of each lane's cell
-    __shared__ uint16_t s_own[kWaves][5 * 64];  // triangle -> lane | (triangle of the cell) << 8
+__device__ __forceinline__ void tri_block(const Src &src, const TriArgs &a, uint32_t tb,
+                                          TriShared &sh) {
+    uint64_t *s_tri = sh.tri;
     if (threadIdx.x < 256) s_tri[threadIdx.x] = a.tri_packed[threadIdx.x];
     __syncthreads();
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = tid & 1;
     {
-        const uint32_t tb = blockIdx.x;
         const uint32_t gb = tb >> 1, part = tb & 1;  // count block and its half
         const uint32_t si = src.seg_of(gb);
         // the block's bookkeeping, loaded with no branch on it so that its
         // round trip is shared with the sign loads': the grid's overflow
         // flag, the block's first triangle and its 8 group counts
         const uint32_t ovf = a.res[si].ovf;
+        const uint64_t v_off = a.res[si].v_off;
         const uint64_t tblock = a.bsum[2 * (uint64_t)gb + 1];
         const uint4 ga = *(const uint4 *)(a.twave + kGroups * (uint64_t)gb);
         const uint4 gc4 = *(const uint4 *)(a.twave + kGroups * (uint64_t)gb + 4);
@@ -698,6 +719,7 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a) {
         const uint32_t *signs = (const uint32_t *)(a.signs + s.sign_off);
         const uint64_t k = (uint64_t)(gb - s.block_off) * kBlock + part * kTriWords + (tid >> 1);
         const uint32_t *base = a.base + (uint64_t)s.block_off * kBlock * 2;  // by grid half word
+        const uint64_t *gbsum = a.bsum + 2 * (uint64_t)s.block_off;       // by grid block
         // ---- per half word (owner lane): the sign half of row (y, z) and the
         // crossing halves of the four rows (y..y+1, z..z+1) touching its
         // cells; a cell's cube index and edge vertex numbers follow from these
@@ -732,13 +754,16 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a) {
                 const uint32_t wf = ldh(signs, i0 + 2 * W2 + RW2, has_y2);
                 const uint32_t wg = ldh(signs, i0 + 2 * RW2, has_z2);
                 const uint32_t wh = ldh(signs, i0 + 2 * RW2 + W2, has_z2);
-                // the first vertex ids are needed only by halves with active
-                // cells, but are loaded (and kept) unconditionally so that
-                // they share the sign loads' round trip
-                vb0 = base[i0];
-                vb1 = base[i0 + W2];
-                vb2 = base[i0 + RW2];
-                vb3 = base[i0 + RW2 + W2];
+                // the first vertex ids (block-local, plus the first vertex
+                // of the block holding the row's word) are needed only by
+                // halves with active cells, but are loaded (and kept)
+                // unconditionally so that they share the sign loads' round trip
+                static_assert(kBlock == 256, "a word's block is its index >> 8");
+                const uint32_t kw = (uint32_t)k, Wd = (uint32_t)g.W, RWd = (uint32_t)g.ny * Wd;
+                vb0 = base[i0] + (uint32_t)(gbsum[2 * (kw >> 8)] - v_off);
+                vb1 = base[i0 + W2] + (uint32_t)(gbsum[2 * ((kw + Wd) >> 8)] - v_off);
+                vb2 = base[i0 + RW2] + (uint32_t)(gbsum[2 * ((kw + RWd) >> 8)] - v_off);
+                vb3 = base[i0 + RW2 + W2] + (uint32_t)(gbsum[2 * ((kw + RWd + Wd) >> 8)] - v_off);
                 const uint32_t sa = sh1(wa, na), sb = sh1(wb, nb), sc = sh1(wc, nc),
                                sd = sh1(wd, nd);
                 act = ((wa ^ sa) | (wa ^ wb) | (wa ^ sb) | (wa ^ wc) | (wa ^ sc) | (wa ^ wd) |
@@ -773,10 +798,10 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a) {
         const uint32_t aincl = wave_scan_incl(nact);
         const uint32_t atot = readlane32(aincl, 63);
         const uint32_t aex = aincl - nact;
-        uint16_t *list = s_list[wave];
-        uint32_t *vid = s_vid[wave];
-        uint64_t *pk = s_pk[wave];
-        uint16_t *own = s_own[wave];
+        uint16_t *list = sh.list[wave];
+        uint32_t *vid = sh.vid[wave];
+        uint64_t *pk = sh.pk[wave];
+        uint16_t *own = sh.own[wave];
         for (uint32_t r0 = 0; r0 < atot; r0 += kCellCap) {
             fill_cells(list, act, nact, aex, atot, r0, lane);
             const uint32_t n = min((uint32_t)kCellCap, atot - r0);
@@ -877,6 +902,28 @@ __global__ void __launch_bounds__(kTriThreads) tri_kernel(Src src, TriArgs a) {
     }
 }
 
+// The vertex and triangle stages in one launch: they depend only on the count
+// stage and the block scan, so their workgroups are interleaved (workgroup w:
+// w % 3 == 0 runs count block w / 3's vertices, the other two its two
+// triangle halves) and run side by side on every CU; the stages wait on
+// different things (field loads, LDS gathers), so each hides the other's
+// latency.
+union EmitShared {
+    VertShared v;
+    TriShared t;
+};
+
+template <class Src>
+__global__ void __launch_bounds__(kBlock) emit_kernel(Src src, VertArgs va, TriArgs ta) {
+    static_assert(kTriThreads == kBlock, "both stages run 256-thread workgroups");
+    __shared__ EmitShared sh;
+    const uint32_t w = blockIdx.x, gb = w / 3, r = w - 3 * gb;
+    if (r == 0)
+        vert_block(src, va, gb, sh.v);
+    else
+        tri_block(src, ta, 2 * gb + (r - 1), sh.t);
+}
+
 }  // namespace
 
 // Enqueues the whole one-grid extraction on `stream` given a workspace.
@@ -926,26 +973,25 @@ int enqueue_extract(const gc_extract_args *a, hipStream_t stream) {
     SegResult *res = (SegResult *)(ws + L.res_off);
     const dim3 blocks((unsigned)L.nb);
     hipLaunchKernelGGL(count_kernel<OneGrid>, blocks, dim3(kBlock), 0, stream, one,
-                       CountArgs{signs, twave, bsum});
+                       CountArgs{signs, twave, bsum, base});
     GC_CHECK_LAUNCH();
     hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, stream, bsum, L.nb, ctl,
                        a->d_counts, (const Seg *)nullptr, 1u, res);
     GC_CHECK_LAUNCH();
-    const VertArgs va = {a->field, signs, bsum, res, base, a->vertices,
+    const VertArgs va = {a->field, signs, bsum, res, a->vertices,
                          a->vertices ? a->vert_capacity : 0};
-    hipLaunchKernelGGL(vert_kernel<OneGrid>, blocks, dim3(kBlock), 0, stream, one, va);
-    GC_CHECK_LAUNCH();
     const TriArgs ta = {signs, base, twave, bsum, tabs->tri_packed, res, a->triangles,
                         a->triangles ? a->tri_capacity : 0};
-    hipLaunchKernelGGL(tri_kernel<OneGrid>, dim3(2 * (unsigned)L.nb), dim3(kTriThreads), 0, stream,
-                       one, ta);
+    GC_ARG(L.nb <= 0xffffffffull / 3, "grid too large");
+    hipLaunchKernelGGL(emit_kernel<OneGrid>, dim3(3 * (unsigned)L.nb), dim3(kBlock), 0, stream,
+                       one, va, ta);
     GC_CHECK_LAUNCH();
     return GPUCAD_OK;
 }
 
 int enqueue_extract_batch(const BatchExtract &b, int phases, hipStream_t stream) {
     GC_ARG(b.nseg > 0 && b.nb > 0 && b.segs && b.blk2seg, "empty batch");
-    GC_ARG(b.nb <= 0x7fffffffu / 2, "batch too large");
+    GC_ARG(b.nb <= 0xffffffffu / 3, "batch too large");
     const DeviceTables *tabs = nullptr;
     int st = device_tables(&tabs);
     if (st) return st;
@@ -954,21 +1000,19 @@ int enqueue_extract_batch(const BatchExtract &b, int phases, hipStream_t stream)
     // one grid
     if (phases & EXTRACT_COUNT) {
         hipLaunchKernelGGL(count_kernel<ManyGrids>, dim3(b.nb), dim3(kBlock), 0, stream, src,
-                           CountArgs{b.signs, b.twave, b.bsum});
+                           CountArgs{b.signs, b.twave, b.bsum, b.base});
         GC_CHECK_LAUNCH();
         hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, stream, b.bsum,
                            (uint64_t)b.nb, b.ctl, (uint64_t *)nullptr, b.segs, b.nseg, b.res);
         GC_CHECK_LAUNCH();
     }
     if (phases & EXTRACT_EMIT) {
-        const VertArgs va = {b.fields, b.signs, b.bsum, b.res, b.base, b.vertices,
+        const VertArgs va = {b.fields, b.signs, b.bsum, b.res, b.vertices,
                              b.vertices ? b.vcap : 0};
-        hipLaunchKernelGGL(vert_kernel<ManyGrids>, dim3(b.nb), dim3(kBlock), 0, stream, src, va);
-        GC_CHECK_LAUNCH();
         const TriArgs ta = {b.signs, b.base, b.twave, b.bsum, tabs->tri_packed, b.res,
                             b.triangles, b.triangles ? b.tcap : 0};
-        hipLaunchKernelGGL(tri_kernel<ManyGrids>, dim3(2 * b.nb), dim3(kTriThreads), 0, stream,
-                           src, ta);
+        hipLaunchKernelGGL(emit_kernel<ManyGrids>, dim3(3 * b.nb), dim3(kBlock), 0, stream, src,
+                           va, ta);
         GC_CHECK_LAUNCH();
     }
     return GPUCAD_OK;

@@ -295,11 +295,20 @@ __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__res
 
     const uint64_t plane = (uint64_t)c.nx * (uint64_t)c.ny;
     const int yrow = yin ? y : 0;  // rows past the grid compute but never store
-    float *out = field + (uint64_t)z0 * plane + (uint64_t)yrow * c.nx + x;
-    const uint64_t sstride = (uint64_t)c.ny * c.sign_words_per_row;
-    uint64_t *srow = signs ? signs + ((uint64_t)z0 * c.ny + yrow) * c.sign_words_per_row : nullptr;
-    const bool sign_lane = srow != nullptr && lx == 0 && yin;
+    // stores address a z plane through a base shared by the workgroup plus a
+    // 32-bit byte offset per lane (a plane of the field and of the sign words
+    // is well below 4 GiB), so the per-plane step is scalar work
+    char *fplane = (char *)(field + (uint64_t)z0 * plane);
+    const uint32_t foff = 4u * ((uint32_t)yrow * (uint32_t)c.nx + (uint32_t)x);
+    const uint64_t fstep = 4 * plane;
+    const uint64_t sstep = 8 * (uint64_t)c.ny * c.sign_words_per_row;
+    char *splane = signs ? (char *)(signs + (uint64_t)z0 * c.ny * c.sign_words_per_row) : nullptr;
+    const uint32_t soff = 8u * ((uint32_t)yrow * (uint32_t)c.sign_words_per_row);
+    const bool sign_lane = signs != nullptr && lx == 0 && yin;
+    // lanes whose point lies in the grid (the sign mask keeps only these)
+    const uint64_t in_mask = __ballot(xin);
 
+#pragma unroll 2
     for (int i = 0; i < zn; ++i) {
         const float *e = ztab + i * kZTab;
         float f;
@@ -365,13 +374,15 @@ __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__res
             g = op == GC_OP_INTERSECT ? gi
                                       : (op == GC_OP_UNION ? gu : (op == GC_OP_SUBTRACT ? gs : g));
         }
-        if (xin && yin) out[(uint64_t)i * plane] = g;
-        if (srow) {
-            const uint64_t m = __ballot(xin && g < c.iso);
+        if (xin && yin) *(float *)(fplane + (uint64_t)i * fstep + foff) = g;
+        if (splane) {
+            // one compare gives the wave's mask of lanes with g < iso (4 is
+            // the ordered less-than predicate: NaN compares false)
+            const uint64_t m = __builtin_amdgcn_fcmpf(g, c.iso, 4) & in_mask;
             if (sign_lane) {
                 const uint64_t bits =
                     BX == 64 ? m : (m >> (ly * BX)) & ((1ull << (BX & 63)) - 1ull);
-                store_sign_bits<BX>(srow + (uint64_t)i * sstride, x0, bits);
+                store_sign_bits<BX>((uint64_t *)(splane + (uint64_t)i * sstep + soff), x0, bits);
             }
         }
     }
