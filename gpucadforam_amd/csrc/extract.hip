@@ -125,6 +125,17 @@ struct T3 {
     uint32_t a, b, c;
 };
 
+// The mesh is written once and not read back by this launch: its 12-byte
+// records are stored with the non-temporal hint (one dwordx3 store each).
+typedef float F3nt __attribute__((ext_vector_type(3), aligned(4)));
+typedef uint32_t U3nt __attribute__((ext_vector_type(3), aligned(4)));
+__device__ __forceinline__ void store_nt(float *p, V3 v) {
+    __builtin_nontemporal_store(F3nt{v.x, v.y, v.z}, (F3nt *)p);
+}
+__device__ __forceinline__ void store_nt(uint32_t *p, T3 t) {
+    __builtin_nontemporal_store(U3nt{t.a, t.b, t.c}, (U3nt *)p);
+}
+
 __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) {
     uint32_t q = (uint32_t)((double)n * f.inv);
     const int64_t r = (int64_t)n - (int64_t)q * f.d;
@@ -593,7 +604,7 @@ __device__ __forceinline__ void vert_block(const Src &src, const VertArgs &a, ui
                     const V3 v = edge_vertex(s, (float)((xy & 0xffff) + b), (float)(xy >> 16),
                                              (float)s_z[j], ax, f0[q], f1[q]);
                     const uint64_t id = vwave + r0 + e;
-                    if (e < n && id < a.vcap) *(V3 *)(a.vertices + 3 * id) = v;
+                    if (e < n && id < a.vcap) store_nt(a.vertices + 3 * id, v);
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -892,7 +903,7 @@ __device__ __forceinline__ void tri_block(const Src &src, const TriArgs &a, uint
                         const uint32_t v0 = vid[e0v * 64 + (l ^ e0v)];
                         const uint32_t v1 = vid[e1v * 64 + (l ^ e1v)];
                         const uint32_t v2 = vid[e2v * 64 + (l ^ e2v)];
-                        *(T3 *)(a.triangles + 3 * t) = T3{v0, v1, v2};
+                        store_nt(a.triangles + 3 * t, T3{v0, v1, v2});
                     }
                 }
                 tnext += ttot;

@@ -24,13 +24,20 @@
 namespace gc {
 namespace {
 
+// Rows per lane of the one-grid launch: a rotated lattice evaluates its z
+// phase (and its cos) per lane and z, so four rows share it; without
+// rotation the z terms come from the brick's table and one row per lane
+// keeps small grids spread over more workgroups.
+constexpr int rows_per_lane(bool rot) { return rot ? 4 : 1; }
+
 template <bool ROT, int TPMS, bool THICK, int PRIM, int OP>
 __global__ void __launch_bounds__(64 * kFieldWaves)
 lattice_field_kernel(FieldConsts c, float *__restrict__ field, uint64_t *__restrict__ signs) {
     __shared__ float ztab[kMaxLZ * kZTab];
-    lattice_brick<ROT, TPMS, THICK, 64, PRIM, OP>(c, field, signs, blockIdx.x * 64,
-                                                  blockIdx.y * kFieldWaves, blockIdx.z * c.lz,
-                                                  ztab);
+    constexpr int RY = rows_per_lane(ROT);
+    lattice_brick<ROT, TPMS, THICK, 64, PRIM, OP, RY>(c, field, signs, blockIdx.x * 64,
+                                                      blockIdx.y * kFieldWaves * RY,
+                                                      blockIdx.z * c.lz, ztab);
 }
 
 typedef void (*FieldKernel)(FieldConsts, float *, uint64_t *);
@@ -95,16 +102,23 @@ int launch_lattice_field(const gc_lattice_params *p, float *d_field, uint64_t *d
     int st = validate_lattice(p);
     if (st) return st;
     GC_ARG(d_field, "field pointer is NULL");
-    // default z run: long enough to amortise the per-column work, short enough
-    // to leave about 8 workgroups per z column of the grid
-    if (lz <= 0) lz = std::max(16, std::min(kMaxLZ, p->nz / 8));
+    const bool rot = p->rot0 != 0.f || p->rot1 != 0.f;
+    const int rows = rows_per_lane(rot);
+    // default z run: long enough to amortise the per-column work (with four
+    // rows per lane that work is four rows' thickness modes, so the run is
+    // about a fifth of the grid, rounded up to 16 planes: 112 at 512^3,
+    // measured best among 32..128), short enough to leave several workgroups
+    // per z column
+    if (lz <= 0)
+        lz = rows > 1 ? std::max(16, std::min(kMaxLZ, ((p->nz + 4) / 5 + 15) / 16 * 16))
+                      : std::max(16, std::min(kMaxLZ, p->nz / 8));
     if (lz > kMaxLZ) lz = kMaxLZ;
     const FieldConsts c = make_field_consts(p, iso, lz);
-    const bool rot = p->rot0 != 0.f || p->rot1 != 0.f;
     const bool thick = p->n_modes > 0;
     FieldKernel k = rot ? pick_tpms<true>(p->tpms, thick, c.prim, c.op)
                         : pick_tpms<false>(p->tpms, thick, c.prim, c.op);
-    dim3 grid((p->nx + 63) / 64, (p->ny + kFieldWaves - 1) / kFieldWaves, (p->nz + lz - 1) / lz);
+    const int brick_y = kFieldWaves * rows;
+    dim3 grid((p->nx + 63) / 64, (p->ny + brick_y - 1) / brick_y, (p->nz + lz - 1) / lz);
     GC_ARG(grid.y <= 65535 && grid.z <= 65535, "grid too large for one launch");
     hipLaunchKernelGGL(k, grid, dim3(64 * kFieldWaves), 0, stream, c, d_field, d_signs);
     GC_CHECK_LAUNCH();

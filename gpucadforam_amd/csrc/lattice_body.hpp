@@ -165,19 +165,27 @@ constexpr int PRIM_ANY = -1;
 // called by all 256 threads of the workgroup (it synchronises through LDS);
 // `ztab` is kMaxLZ * kZTab floats of LDS.  `field` and `signs` point at the
 // grid's own first point and sign word; `signs` may be NULL.  PRIM and OP are
-// c.prim and c.op, or PRIM_ANY (read at run time).  The per-point loop has no data-independent branches: absent
-// thickness groups have zero coefficients (their fmas add exactly 0) and the
-// boolean is a select among max(g, s), min(g, s), max(g, -s).
-template <bool ROT, int TPMS, bool THICK, int BX, int PRIM = PRIM_ANY, int OP = PRIM_ANY>
+// c.prim and c.op, or PRIM_ANY (read at run time).  The per-point loop has no
+// data-independent branches: absent thickness groups have zero coefficients
+// (their fmas add exactly 0) and the boolean is a select among max(g, s),
+// min(g, s), max(g, -s).
+//
+// RY > 1 (one-grid launches, BX = 64): each lane owns RY consecutive rows of
+// its x column, so the brick is 64 x 4 RY columns and the work that depends
+// on (x, z) only (the z table reads, the z phase and its cos / sin) is done
+// once per RY points.  Every point's arithmetic is the same for any RY.
+template <bool ROT, int TPMS, bool THICK, int BX, int PRIM = PRIM_ANY, int OP = PRIM_ANY,
+          int RY = 1>
 __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__restrict__ field,
                                               uint64_t *__restrict__ signs, int x0, int y0,
                                               int z0, float *ztab) {
+    static_assert(RY == 1 || BX == 64, "rows per lane need one row per wave");
     constexpr int BY = 64 / BX;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int lx = lane % BX, ly = lane / BX;
     const int x = x0 + lx;
-    const int y = y0 + wave * BY + ly;
+    const int yb = y0 + (wave * BY + ly) * RY;  // the lane's first row
     const int zn = min(c.lz, c.nz - z0);
 
     // ---- per-z table (one z per thread)
@@ -210,179 +218,226 @@ __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__res
         }
     }
     __syncthreads();
-    const bool yin = y < c.ny;
     const bool xin = x < c.nx;
 
-    // ---- per-lane (x, y) quantities
-    const float xf = (float)x, yf = (float)y;
-    const float px = xf * c.hx, py = yf * c.hy;
+    // ---- per-lane x quantities
+    const float xf = (float)x;
+    const float px = xf * c.hx;
     const float period = __builtin_fmaf(c.dp, xf, c.p0);
     const float inv_p = 1.0f / period;
-    float dxP = 0.f, dyP = 0.f, cxP = 0.f, cyP = 0.f;
-    float hA = 0.f, hB = 0.f, cX = 0.f, sX = 0.f;  // hoisted TPMS partials (non-rotated)
+    float dxP = 0.f, cxP = 0.f, cyP = 0.f;
     if (ROT) {
         dxP = (xf - c.cx) * inv_p;
-        dyP = (yf - c.cy) * inv_p;
         cxP = c.cx * inv_p;
         cyP = c.cy * inv_p;
-    } else {
-        float X, Y;
-        if (c.bend) {
-            float ax = px - c.bend_cx, ay = py - c.bend_cy;
-            float r = __builtin_sqrtf(__builtin_fmaf(ay, ay, ax * ax));
-            float phi = atan2_poly(ax, ay);
-            float ud = __builtin_fmaf(c.bend_r, phi, c.bend_cx);
-            float vd = c.bend_cy + r;
-            X = (ud / c.hx) * inv_p;
-            Y = (vd / c.hy) * inv_p;
-        } else {
-            X = xf * inv_p;
-            Y = yf * inv_p;
-        }
-        sX = sin_rev(X);
-        cX = cos_rev(X);
-        float sY = sin_rev(Y), cY = cos_rev(Y);
-        if (TPMS == GC_TPMS_GYROID) {
-            hA = sX * cY;  // sX cY
-            hB = sY;       // sY (times cZ)
-        } else if (TPMS == GC_TPMS_SCHWARZ_P) {
-            hA = cX + cY;
-        } else {
-            hA = __builtin_fmaf(sX, sY, cX * cY);  // multiplies sZ
-            hB = __builtin_fmaf(sX, cY, cX * sY);  // multiplies cZ
-        }
     }
-    // thickness: S_k = sum over modes with kz == k of a_m e^{2 pi i phi_m} Ex Ey
-    float T0 = c.t0;
-    Cplx S1 = {0.f, 0.f}, S2 = {0.f, 0.f}, S3 = {0.f, 0.f};
+    Cplx ex1 = {1.f, 0.f}, ex2 = ex1, ex3 = ex1;
     if (THICK) {
-        // e^{2 pi i k px} and e^{2 pi i k py} for k = 0..3, selected without
-        // dynamic register indexing (which would spill to scratch)
-        const Cplx ex1 = expi_rev(px), ex2 = expi_rev(2.0f * px), ex3 = expi_rev(3.0f * px);
-        const Cplx ey1 = expi_rev(py), ey2 = expi_rev(2.0f * py), ey3 = expi_rev(3.0f * py);
-        auto pick = [](int k, Cplx e1, Cplx e2, Cplx e3) {
-            Cplx r = {1.f, 0.f};
-            r = k == 1 ? e1 : r;
-            r = k == 2 ? e2 : r;
-            r = k == 3 ? e3 : r;
-            return r;
-        };
-        Cplx S0 = {0.f, 0.f};
-        for (int m = 0; m < c.n_modes; ++m) {
-            Cplx w = cmul(pick(c.mode_k[m][0], ex1, ex2, ex3), pick(c.mode_k[m][1], ey1, ey2, ey3));
-            Cplx ph = {c.phi_re[m], c.phi_im[m]};
-            Cplx a = cmul(w, ph);
-            int kz = c.mode_k[m][2];
-            if (kz == 0) { S0.re += a.re; S0.im += a.im; }
-            if (kz == 1) { S1.re += a.re; S1.im += a.im; }
-            if (kz == 2) { S2.re += a.re; S2.im += a.im; }
-            if (kz == 3) { S3.re += a.re; S3.im += a.im; }
-        }
-        T0 = c.t0 + S0.re;
+        ex1 = expi_rev(px);
+        ex2 = expi_rev(2.0f * px);
+        ex3 = expi_rev(3.0f * px);
     }
-    // primitive partials
-    float pxy = 0.f, pbxy = 0.f;
-    if (c.prim == GC_PRIM_SPHERE) {
-        float dx = px - c.pc[0], dy = py - c.pc[1];
-        pxy = __builtin_fmaf(dy, dy, dx * dx);
-    } else if (c.prim == GC_PRIM_BOX) {
-        float bx = __builtin_fabsf(px - c.pc[0]) - c.ph[0];
-        float by = __builtin_fabsf(py - c.pc[1]) - c.ph[1];
-        float mx = __builtin_fmaxf(bx, 0.f), my = __builtin_fmaxf(by, 0.f);
-        pxy = __builtin_fmaf(my, my, mx * mx);
-        pbxy = __builtin_fmaxf(bx, by);
+    // ---- per-row (x, y) quantities
+    bool yin[RY];
+    float dyP[RY], hA[RY], hB[RY], cX[RY];  // hoisted TPMS partials (non-rotated)
+    float T0[RY];
+    Cplx S1[RY], S2[RY], S3[RY];
+    float pxy[RY], pbxy[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const int y = yb + r;
+        yin[r] = y < c.ny;
+        const float yf = (float)y;
+        const float py = yf * c.hy;
+        dyP[r] = hA[r] = hB[r] = cX[r] = 0.f;
+        if (ROT) {
+            dyP[r] = (yf - c.cy) * inv_p;
+        } else {
+            float X, Y;
+            if (c.bend) {
+                float ax = px - c.bend_cx, ay = py - c.bend_cy;
+                float rr = __builtin_sqrtf(__builtin_fmaf(ay, ay, ax * ax));
+                float phi = atan2_poly(ax, ay);
+                float ud = __builtin_fmaf(c.bend_r, phi, c.bend_cx);
+                float vd = c.bend_cy + rr;
+                X = (ud / c.hx) * inv_p;
+                Y = (vd / c.hy) * inv_p;
+            } else {
+                X = xf * inv_p;
+                Y = yf * inv_p;
+            }
+            const float sX = sin_rev(X);
+            cX[r] = cos_rev(X);
+            float sY = sin_rev(Y), cY = cos_rev(Y);
+            if (TPMS == GC_TPMS_GYROID) {
+                hA[r] = sX * cY;  // sX cY
+                hB[r] = sY;       // sY (times cZ)
+            } else if (TPMS == GC_TPMS_SCHWARZ_P) {
+                hA[r] = cX[r] + cY;
+            } else {
+                hA[r] = __builtin_fmaf(sX, sY, cX[r] * cY);  // multiplies sZ
+                hB[r] = __builtin_fmaf(sX, cY, cX[r] * sY);  // multiplies cZ
+            }
+        }
+        // thickness: S_k = sum over modes with kz == k of a_m e^{2 pi i phi_m} Ex Ey
+        T0[r] = c.t0;
+        S1[r] = S2[r] = S3[r] = Cplx{0.f, 0.f};
+        if (THICK) {
+            // e^{2 pi i k py} for k = 0..3, selected without dynamic register
+            // indexing (which would spill to scratch)
+            const Cplx ey1 = expi_rev(py), ey2 = expi_rev(2.0f * py), ey3 = expi_rev(3.0f * py);
+            auto pick = [](int k, Cplx e1, Cplx e2, Cplx e3) {
+                Cplx q = {1.f, 0.f};
+                q = k == 1 ? e1 : q;
+                q = k == 2 ? e2 : q;
+                q = k == 3 ? e3 : q;
+                return q;
+            };
+            Cplx S0 = {0.f, 0.f};
+            for (int m = 0; m < c.n_modes; ++m) {
+                Cplx w = cmul(pick(c.mode_k[m][0], ex1, ex2, ex3),
+                              pick(c.mode_k[m][1], ey1, ey2, ey3));
+                Cplx ph = {c.phi_re[m], c.phi_im[m]};
+                Cplx a = cmul(w, ph);
+                int kz = c.mode_k[m][2];
+                if (kz == 0) { S0.re += a.re; S0.im += a.im; }
+                if (kz == 1) { S1[r].re += a.re; S1[r].im += a.im; }
+                if (kz == 2) { S2[r].re += a.re; S2[r].im += a.im; }
+                if (kz == 3) { S3[r].re += a.re; S3[r].im += a.im; }
+            }
+            T0[r] = c.t0 + S0.re;
+        }
+        // primitive partials
+        pxy[r] = pbxy[r] = 0.f;
+        if (c.prim == GC_PRIM_SPHERE) {
+            float dx = px - c.pc[0], dy = py - c.pc[1];
+            pxy[r] = __builtin_fmaf(dy, dy, dx * dx);
+        } else if (c.prim == GC_PRIM_BOX) {
+            float bx = __builtin_fabsf(px - c.pc[0]) - c.ph[0];
+            float by = __builtin_fabsf(py - c.pc[1]) - c.ph[1];
+            float mx = __builtin_fmaxf(bx, 0.f), my = __builtin_fmaxf(by, 0.f);
+            pxy[r] = __builtin_fmaf(my, my, mx * mx);
+            pbxy[r] = __builtin_fmaxf(bx, by);
+        }
     }
 
     const uint64_t plane = (uint64_t)c.nx * (uint64_t)c.ny;
-    const int yrow = yin ? y : 0;  // rows past the grid compute but never store
     // stores address a z plane through a base shared by the workgroup plus a
     // 32-bit byte offset per lane (a plane of the field and of the sign words
-    // is well below 4 GiB), so the per-plane step is scalar work
+    // is well below 4 GiB); rows past the grid compute but never store
     char *fplane = (char *)(field + (uint64_t)z0 * plane);
-    const uint32_t foff = 4u * ((uint32_t)yrow * (uint32_t)c.nx + (uint32_t)x);
     const uint64_t fstep = 4 * plane;
     const uint64_t sstep = 8 * (uint64_t)c.ny * c.sign_words_per_row;
     char *splane = signs ? (char *)(signs + (uint64_t)z0 * c.ny * c.sign_words_per_row) : nullptr;
-    const uint32_t soff = 8u * ((uint32_t)yrow * (uint32_t)c.sign_words_per_row);
-    const bool sign_lane = signs != nullptr && lx == 0 && yin;
+    uint32_t foff[RY], soff[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const uint32_t yrow = yin[r] ? (uint32_t)(yb + r) : 0u;
+        foff[r] = 4u * (yrow * (uint32_t)c.nx + (uint32_t)x);
+        soff[r] = 8u * (yrow * (uint32_t)c.sign_words_per_row);
+    }
+    const bool sign_lane = signs != nullptr && lx == 0;
     // lanes whose point lies in the grid (the sign mask keeps only these)
     const uint64_t in_mask = __ballot(xin);
 
-#pragma unroll 2
+    constexpr int kZUnroll = RY == 1 ? 2 : 1;
+#pragma unroll kZUnroll
     for (int i = 0; i < zn; ++i) {
         const float *e = ztab + i * kZTab;
-        float f;
+        // (x, z) work, shared by the lane's rows
+        float cth = 0.f, sth = 0.f, cZ = 0.f, sZ = 0.f, Zr = 0.f;
         if (ROT) {
-            const float zf = e[ZT_ZF];
-            const float cth = e[ZT_CTH], sth = e[ZT_STH];
-            float X = __builtin_fmaf(cth, dxP, __builtin_fmaf(-sth, dyP, cxP));
-            float Y = __builtin_fmaf(sth, dxP, __builtin_fmaf(cth, dyP, cyP));
-            float Z = zf * inv_p;
+            cth = e[ZT_CTH];
+            sth = e[ZT_STH];
+            Zr = e[ZT_ZF] * inv_p;
             if (TPMS == GC_TPMS_SCHWARZ_P) {
-                f = (cos_rev(X) + cos_rev(Y)) + cos_rev(Z);
+                cZ = cos_rev(Zr);
             } else {
-                float sx = sin_rev(X), cx = cos_rev(X), sy = sin_rev(Y), cy = cos_rev(Y);
-                float sz = sin_rev(Z), cz = cos_rev(Z);
-                if (TPMS == GC_TPMS_GYROID)
-                    f = __builtin_fmaf(sz, cx, __builtin_fmaf(sy, cz, sx * cy));
-                else
-                    f = __builtin_fmaf(sz, __builtin_fmaf(sx, sy, cx * cy),
-                                       cz * __builtin_fmaf(sx, cy, cx * sy));
+                sZ = sin_rev(Zr);
+                cZ = cos_rev(Zr);
             }
         } else {
-            float cz, sz;
             if (c.uniform_period) {
-                cz = e[ZT_CZ];
-                sz = e[ZT_SZ];
+                cZ = e[ZT_CZ];
+                sZ = e[ZT_SZ];
             } else {
-                float Z = e[ZT_ZF] * inv_p;
-                cz = cos_rev(Z);
-                sz = sin_rev(Z);
+                const float Z = e[ZT_ZF] * inv_p;
+                cZ = cos_rev(Z);
+                sZ = sin_rev(Z);
             }
-            if (TPMS == GC_TPMS_GYROID)
-                f = __builtin_fmaf(sz, cX, __builtin_fmaf(hB, cz, hA));
-            else if (TPMS == GC_TPMS_SCHWARZ_P)
-                f = hA + cz;
-            else
-                f = __builtin_fmaf(sz, hA, cz * hB);
         }
-        float t = T0;
+        float E[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (THICK) {
-            // a group without modes has S = 0: its fmas add exactly 0
-            t = __builtin_fmaf(S1.re, e[ZT_E1 + 0], t);
-            t = __builtin_fmaf(-S1.im, e[ZT_E1 + 1], t);
-            t = __builtin_fmaf(S2.re, e[ZT_E1 + 2], t);
-            t = __builtin_fmaf(-S2.im, e[ZT_E1 + 3], t);
-            t = __builtin_fmaf(S3.re, e[ZT_E1 + 4], t);
-            t = __builtin_fmaf(-S3.im, e[ZT_E1 + 5], t);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) E[q] = e[ZT_E1 + q];
         }
-        float g = c.sheet ? __builtin_fabsf(f) - t : f;
         const int prim = PRIM == PRIM_ANY ? c.prim : PRIM;
-        if (prim != GC_PRIM_NONE) {
-            float s;
-            const float pz = e[ZT_PRIMZ];
-            if (prim == GC_PRIM_SPHERE) {
-                s = sqrt_rn(__builtin_fmaf(pz, pz, pxy)) - c.pr;
+        const float pz = prim != GC_PRIM_NONE ? e[ZT_PRIMZ] : 0.f;
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            float f;
+            if (ROT) {
+                float X = __builtin_fmaf(cth, dxP, __builtin_fmaf(-sth, dyP[r], cxP));
+                float Y = __builtin_fmaf(sth, dxP, __builtin_fmaf(cth, dyP[r], cyP));
+                if (TPMS == GC_TPMS_SCHWARZ_P) {
+                    f = (cos_rev(X) + cos_rev(Y)) + cZ;
+                } else {
+                    float sx = sin_rev(X), cx = cos_rev(X), sy = sin_rev(Y), cy = cos_rev(Y);
+                    if (TPMS == GC_TPMS_GYROID)
+                        f = __builtin_fmaf(sZ, cx, __builtin_fmaf(sy, cZ, sx * cy));
+                    else
+                        f = __builtin_fmaf(sZ, __builtin_fmaf(sx, sy, cx * cy),
+                                           cZ * __builtin_fmaf(sx, cy, cx * sy));
+                }
             } else {
-                float mz = __builtin_fmaxf(pz, 0.f);
-                s = sqrt_rn(__builtin_fmaf(mz, mz, pxy)) +
-                    __builtin_fminf(__builtin_fmaxf(pbxy, pz), 0.f);
+                if (TPMS == GC_TPMS_GYROID)
+                    f = __builtin_fmaf(sZ, cX[r], __builtin_fmaf(hB[r], cZ, hA[r]));
+                else if (TPMS == GC_TPMS_SCHWARZ_P)
+                    f = hA[r] + cZ;
+                else
+                    f = __builtin_fmaf(sZ, hA[r], cZ * hB[r]);
             }
-            const int op = OP == PRIM_ANY ? c.op : OP;
-            const float gi = __builtin_fmaxf(g, s), gu = __builtin_fminf(g, s),
-                        gs = __builtin_fmaxf(g, -s);
-            g = op == GC_OP_INTERSECT ? gi
-                                      : (op == GC_OP_UNION ? gu : (op == GC_OP_SUBTRACT ? gs : g));
-        }
-        if (xin && yin) *(float *)(fplane + (uint64_t)i * fstep + foff) = g;
-        if (splane) {
-            // one compare gives the wave's mask of lanes with g < iso (4 is
-            // the ordered less-than predicate: NaN compares false)
-            const uint64_t m = __builtin_amdgcn_fcmpf(g, c.iso, 4) & in_mask;
-            if (sign_lane) {
-                const uint64_t bits =
-                    BX == 64 ? m : (m >> (ly * BX)) & ((1ull << (BX & 63)) - 1ull);
-                store_sign_bits<BX>((uint64_t *)(splane + (uint64_t)i * sstep + soff), x0, bits);
+            float t = T0[r];
+            if (THICK) {
+                // a group without modes has S = 0: its fmas add exactly 0
+                t = __builtin_fmaf(S1[r].re, E[0], t);
+                t = __builtin_fmaf(-S1[r].im, E[1], t);
+                t = __builtin_fmaf(S2[r].re, E[2], t);
+                t = __builtin_fmaf(-S2[r].im, E[3], t);
+                t = __builtin_fmaf(S3[r].re, E[4], t);
+                t = __builtin_fmaf(-S3[r].im, E[5], t);
+            }
+            float g = c.sheet ? __builtin_fabsf(f) - t : f;
+            if (prim != GC_PRIM_NONE) {
+                float s;
+                if (prim == GC_PRIM_SPHERE) {
+                    s = sqrt_rn(__builtin_fmaf(pz, pz, pxy[r])) - c.pr;
+                } else {
+                    float mz = __builtin_fmaxf(pz, 0.f);
+                    s = sqrt_rn(__builtin_fmaf(mz, mz, pxy[r])) +
+                        __builtin_fminf(__builtin_fmaxf(pbxy[r], pz), 0.f);
+                }
+                const int op = OP == PRIM_ANY ? c.op : OP;
+                const float gi = __builtin_fmaxf(g, s), gu = __builtin_fminf(g, s),
+                            gs = __builtin_fmaxf(g, -s);
+                g = op == GC_OP_INTERSECT
+                        ? gi
+                        : (op == GC_OP_UNION ? gu : (op == GC_OP_SUBTRACT ? gs : g));
+            }
+            // the field is written once per launch (the extraction reads it
+            // back from HBM): non-temporal stores
+            if (xin && yin[r])
+                __builtin_nontemporal_store(g, (float *)(fplane + (uint64_t)i * fstep + foff[r]));
+            if (splane) {
+                // one compare gives the wave's mask of lanes with g < iso (4 is
+                // the ordered less-than predicate: NaN compares false)
+                const uint64_t m = __builtin_amdgcn_fcmpf(g, c.iso, 4) & in_mask;
+                if (sign_lane && yin[r]) {
+                    const uint64_t bits =
+                        BX == 64 ? m : (m >> (ly * BX)) & ((1ull << (BX & 63)) - 1ull);
+                    store_sign_bits<BX>((uint64_t *)(splane + (uint64_t)i * sstep + soff[r]), x0,
+                                        bits);
+                }
             }
         }
     }
