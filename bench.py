@@ -103,6 +103,75 @@ def parse():
     return ap.parse_args()
 
 
+class HipEvent:
+    """A HIP timing event created with hipEventDisableSystemFence: recording
+    it writes a timestamp once the stream's earlier work is done, without the
+    system-scope cache writeback and invalidation of an ordinary event, so
+    events between the passes of a step time the passes without lengthening
+    them.  (The process has one HIP runtime, torch's, which libgpucad.so
+    binds too.)"""
+    _hip = None
+    DISABLE_SYSTEM_FENCE = 0x20000000
+
+    def __init__(self):
+        if HipEvent._hip is None:
+            h = C.CDLL("libamdhip64.so.7")
+            h.hipEventCreateWithFlags.argtypes = [C.POINTER(C.c_void_p), C.c_uint]
+            h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+            h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+            h.hipEventDestroy.argtypes = [C.c_void_p]
+            HipEvent._hip = h
+        self.e = C.c_void_p()
+        st = self._hip.hipEventCreateWithFlags(C.byref(self.e), self.DISABLE_SYSTEM_FENCE)
+        if st != 0:
+            raise RuntimeError("hipEventCreateWithFlags failed: %d" % st)
+
+    def record(self, stream_handle):
+        st = self._hip.hipEventRecord(self.e, stream_handle)
+        if st != 0:
+            raise RuntimeError("hipEventRecord failed: %d" % st)
+
+    def ms_to(self, other):
+        """Milliseconds from this event to `other` (both complete)."""
+        f = C.c_float()
+        st = self._hip.hipEventElapsedTime(C.byref(f), self.e, other.e)
+        if st != 0:
+            raise RuntimeError("hipEventElapsedTime failed: %d" % st)
+        return f.value
+
+    def __del__(self):
+        if self._hip is not None and self.e:
+            self._hip.hipEventDestroy(self.e)
+
+
+def timed_steps(pipe, stream, steps):
+    """(ms per step, field ms, extraction ms): first `steps` steps bracketed
+    by two events only, then as many again with a fenceless event before and
+    after each pass for the per-pass times."""
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(steps):
+        pipe.field_pass()
+        pipe.extract_pass()
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / steps
+    ev = [[HipEvent() for _ in range(3)] for _ in range(steps)]
+    h = pipe.h
+    for k in range(steps):
+        ev[k][0].record(h)
+        pipe.field_pass()
+        ev[k][1].record(h)
+        pipe.extract_pass()
+        ev[k][2].record(h)
+    torch.cuda.synchronize()
+    field_ms = sum(e[0].ms_to(e[1]) for e in ev) / steps
+    extract_ms = sum(e[1].ms_to(e[2]) for e in ev) / steps
+    return ms, field_ms, extract_ms
+
+
 class Pipeline:
     def __init__(self, p, stream):
         self.p = p
@@ -181,28 +250,17 @@ def bench_pipelined(p, steps, warmup, depth):
 
 def bench_config(p, name, steps, warmup):
     """Field + indexed extraction of one more single-grid workload (same
-    timing as the headline step: HIP events around each pass)."""
+    timing as the headline step)."""
     stream = torch.cuda.Stream()
     pipe = Pipeline(p, stream)
     for _ in range(warmup):
         pipe.field_pass()
         pipe.extract_pass()
-    torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
-    for k in range(steps):
-        ev[k][0].record(stream)
-        pipe.field_pass()
-        ev[k][1].record(stream)
-        pipe.extract_pass()
-        ev[k][2].record(stream)
-    torch.cuda.synchronize()
-    field_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
-    extract_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / steps
+    ms, field_ms, extract_ms = timed_steps(pipe, stream, steps)
     counts = pipe.counts.cpu().tolist()
     npts = p.npoints()
     flops = FLOP_PER_POINT[name] * npts
     ext_bytes = 4 * npts + 12 * counts[0] + 12 * counts[1]
-    ms = field_ms + extract_ms
     out = {"workload": name, "grid": [p.nx, p.ny, p.nz], "steps": steps,
            "ms_per_step": round(ms, 4),
            "gvoxels_per_s": round(npts / (ms * 1e-3) / 1e9, 3),
@@ -642,29 +700,14 @@ def main():
         pipe.field_pass()
         pipe.extract_pass()
     torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
     t_wall = time.perf_counter()
-    start.record(stream)
-    for k in range(args.steps):
-        ev[k][0].record(stream)
-        pipe.field_pass()
-        ev[k][1].record(stream)
-        pipe.extract_pass()
-        ev[k][2].record(stream)
-    end.record(stream)
-    torch.cuda.synchronize()
+    ms, field_ms, extract_ms = timed_steps(pipe, stream, args.steps)
     wall = time.perf_counter() - t_wall
     if args.calib:
         src = torch.empty(128 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-    ms = start.elapsed_time(end) / args.steps
-    field_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / args.steps
-    extract_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / args.steps
     counts = pipe.counts.cpu().tolist()
     npts = p.npoints()
     flops = FLOP_PER_POINT["varying_schwarz_p"] * npts
