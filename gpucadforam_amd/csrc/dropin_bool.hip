@@ -153,6 +153,50 @@ __global__ void bool_classify_kernel(uint32_t *verts, uint32_t *occ, Vols v, BGr
     occ[i] = nv > 0;
 }
 
+// bit p of bits = the point's occupancy (the header's table), for the
+// orchestration's classify from bits; each wave sets 64 x kWords
+// consecutive bits (the layout dropin_scratch sizes)
+__global__ void __launch_bounds__(256) bool_point_bits_kernel(uint64_t *bits, Vols v, uint32_t np,
+                                                              float iso, float lo, float hi,
+                                                              uint32_t flags) {
+    constexpr int kWords = kDropinBitsWaveWords;
+    const Needs nn = needs(flags);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint64_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < kWords; ++j) {
+        const uint64_t i = wave * (64 * kWords) + j * 64 + lane;
+        uint32_t b = 0;
+        if (i < np) {
+            const float fv = nn.f ? (float)v.fixed[i].val : 0.f;
+            const float d = nn.d ? v.dyn[i] : 0.f;
+            const float l = nn.l ? v.lat[i] : 0.f;
+            b = occupied(flags, fv, d, l, iso, lo, hi);
+        }
+        const uint64_t m = __ballot(b);
+        if (lane == (uint32_t)j) mine = m;
+    }
+    if (lane < kWords) bits[wave * kWords + lane] = mine;
+}
+
+// src/Isosurface.cu:120-121: bytes bytes of pos and norm cleared once some
+// voxel is active (the count is on the device)
+__global__ void clear_if_active_kernel(uint8_t *pos, uint8_t *norm, uint64_t bytes,
+                                       const uint32_t *tot) {
+    if (tot[0] == 0) return;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t n16 = bytes / 16;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) {
+        ((uint4 *)pos)[i] = make_uint4(0, 0, 0, 0);
+        ((uint4 *)norm)[i] = make_uint4(0, 0, 0, 0);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < bytes % 16) {
+        pos[n16 * 16 + threadIdx.x] = 0;
+        norm[n16 * 16 + threadIdx.x] = 0;
+    }
+}
+
 // ---- classify_copy_Voxel: occupancy as val = -1 / 1 and averaged edge parameters
 __device__ __forceinline__ bool crosses(float a, float b, float level) {
     // (a < level && b >= level) || (a >= level && b < level), a = neighbour, b = own
@@ -286,109 +330,163 @@ struct BGenArgs {
     uint32_t active, max_verts, flags;
     const uint8_t *tri;
     const uint8_t *nverts;
+    const uint32_t *active_dev = nullptr;  // when set, the active voxel count is read here
+    const uint32_t *vbase = nullptr;       // when set, scan[comp[i]] is read as vbase[i]
+    const uint32_t *bits = nullptr;        // when set, cube indices come from the point bits
 };
 
 __device__ __forceinline__ float tparam(const GridPoint &p, int ax) {
     return ax == 0 ? p.t_x : (ax == 1 ? p.t_y : p.t_z);
 }
 
+typedef float F4nt __attribute__((ext_vector_type(4)));
+
 // One lane per active voxel computes its 12 edge vertices into LDS as the
 // reference's thread does; the wave then writes its voxels' vertices one
 // output vertex per lane (contiguous with the scan of the voxels' counts, so
-// the stores are coalesced), vertex j of a voxel at scan[voxel] + j, each
-// triangle only if its first index is below maxVerts - 3.
+// the stores are coalesced; non-temporal, nothing reads them back here),
+// vertex j of a voxel at scan[voxel] + j, each triangle only if its first
+// index is below maxVerts - 3.  Batches of 64 active voxels as in the
+// lattice generate: one workgroup per batch, or the orchestration's
+// persistent launch with a contiguous eighth of the batches per XCD.
 __global__ void __launch_bounds__(64) bool_generate_kernel(BGenArgs a) {
-    __shared__ float3 vl[12][64];
+    __shared__ float3 vl[12][65];  // (stride 65: the edges of one voxel in different banks)
     __shared__ uint8_t owner[64 * 15];  // output vertex -> lane
     __shared__ uint32_t s_first[64], s_base[64];
     __shared__ uint4 s_tri[64];  // each lane's triTable row
     const uint32_t lane = threadIdx.x;
-    const uint32_t i = blockIdx.x * 64u + lane;
-    uint32_t cube = 0, nv = 0, vbase = 0;
-    if (i < a.active) {
-        const uint32_t voxel = a.comp[i];
-        vbase = a.scan[voxel];
-        const uint3 gp = calc_grid_pos(voxel, a.g.shift, a.g.mask);
-        float3 p;
-        p.x = ((float)gp.x - a.g.center.x) * a.g.vs.x;
-        p.y = ((float)gp.y - a.g.center.y) * a.g.vs.y;
-        p.z = ((float)gp.z - a.g.center.z) * a.g.vs.z;
-        const float3 vs = a.g.vs;
-        float3 v[8];
-        v[0] = p;
-        v[1] = make_float3(p.x + vs.x, p.y + 0.f, p.z + 0.f);
-        v[2] = make_float3(p.x + vs.x, p.y + vs.y, p.z + 0.f);
-        v[3] = make_float3(p.x + 0.f, p.y + vs.y, p.z + 0.f);
-        v[4] = make_float3(p.x + 0.f, p.y + 0.f, p.z + vs.z);
-        v[5] = make_float3(p.x + vs.x, p.y + 0.f, p.z + vs.z);
-        v[6] = make_float3(p.x + vs.x, p.y + vs.y, p.z + vs.z);
-        v[7] = make_float3(p.x + 0.f, p.y + vs.y, p.z + vs.z);
-        const Needs nn = needs(a.flags);
-        const bool nf = nn.f, nd = nn.d, nl = nn.l;
-        uint32_t idx[8];
-        corner_index(gp, a.g.size, idx);
-        cube = cube_index(a.flags, a.v, idx, a.iso, a.lo, a.hi, nf, nd, nl);
-        nv = a.nverts[cube];
-        const uint32_t f = a.flags;
-        if (nv == 0) {
-            // nothing to write (src/MarchingCubes_kernel.cu:2078)
-        } else if (f & GC_BOOL_MAKE_REGION) {
-#pragma unroll
-            for (int e = 0; e < 12; ++e)
-                vl[e][lane] =
-                    lerp3(v[c_pa[e]], v[c_pb[e]], tparam(a.v.fixed[idx[c_pa[e]]], c_pax[e]));
-        } else if (f & GC_BOOL_FIXED) {
-#pragma unroll
-            for (int e = 0; e < 12; ++e) {
-                const int ca = c_na[e], cb = c_nb[e];
-                vl[e][lane] = interp_new(a.iso, a.lo, a.hi, v[ca], v[cb], a.v.dyn[idx[ca]],
-                                         a.v.dyn[idx[cb]], a.v.lat[idx[ca]], a.v.lat[idx[cb]]);
-            }
-        } else if (f & GC_BOOL_DYNAMIC) {
-#pragma unroll
-            for (int e = 0; e < 12; ++e) {
-                const int ca = c_pa[e], cb = c_pb[e];
-                vl[e][lane] =
-                    interp_primitive_one(a.lo, a.hi, v[ca], v[cb], a.v.lat[idx[ca]],
-                                         a.v.lat[idx[cb]], tparam(a.v.fixed[idx[ca]], c_pax[e]));
-            }
-        } else {
-#pragma unroll
-            for (int e = 0; e < 12; ++e) {
-                const int ca = c_pa[e], cb = c_pb[e];
-                vl[e][lane] =
-                    interp_primitive(a.iso, v[ca], v[cb], a.v.dyn[idx[ca]], a.v.dyn[idx[cb]],
-                                     tparam(a.v.fixed[idx[ca]], c_pax[e]));
-            }
-        }
+    const uint32_t active = a.active_dev ? *a.active_dev : a.active;
+    const uint32_t nbat = (uint32_t)(((uint64_t)active + 63) / 64);
+    uint32_t bat = blockIdx.x, bat_end = nbat, bat_step = gridDim.x;
+    if (a.active_dev) {
+        const uint32_t per = (nbat + 7) / 8, x = blockIdx.x & 7u;
+        bat = x * per + (blockIdx.x >> 3);
+        bat_end = min(nbat, x * per + per);
+        bat_step = gridDim.x >> 3;
     }
-    const uint32_t incl = wave_inclusive_scan(nv);
-    const uint32_t first = incl - nv;
-    const uint32_t total = __shfl(incl, 63, 64);
-    s_first[lane] = first;
-    s_base[lane] = vbase;
-    s_tri[lane] = *(const uint4 *)(a.tri + cube * 16);
-    for (uint32_t j = 0; j < nv; ++j) owner[first + j] = (uint8_t)lane;
-    __syncthreads();
-    for (uint32_t k = lane; k < total; k += 64) {
-        const uint32_t o = owner[k];
-        const uint32_t j = k - s_first[o];
-        const uint32_t t0 = j - j % 3u;  // the triangle's first vertex
-        const uint8_t *row = (const uint8_t *)&s_tri[o];
-        const float3 a0 = vl[row[t0]][o];
-        const float3 a1 = vl[row[t0 + 1]][o];
-        const float3 a2 = vl[row[t0 + 2]][o];
-        // calcNormal(ver[0], ver[2], ver[1]) = cross(v2 - v0, v1 - v0)
-        const float3 e0 = make_float3(a2.x - a0.x, a2.y - a0.y, a2.z - a0.z);
-        const float3 e1 = make_float3(a1.x - a0.x, a1.y - a0.y, a1.z - a0.z);
-        const float3 n = cross3(e0, e1);
-        const uint32_t index = s_base[o] + t0;  // scan[voxel] + t0, uint arithmetic
-        if (index < a.max_verts - 3u) {
-            const uint32_t r = j - t0;
-            const float3 q = r == 0 ? a0 : (r == 1 ? a1 : a2);
-            a.pos[index + r] = make_float4(q.x, q.y, q.z, 1.0f);
-            a.norm[index + r] = make_float4(n.x, n.y, n.z, 0.5f);
+    const Needs nn = needs(a.flags);
+    const bool nf = nn.f, nd = nn.d, nl = nn.l;
+    // each batch's voxel ids and vertex offsets are read one batch ahead
+    uint32_t next_voxel = 0, next_vbase = 0;
+    auto fetch = [&](uint32_t b) {
+        const uint64_t j = (uint64_t)b * 64u + lane;
+        if (b < bat_end && j < active) {
+            next_voxel = a.comp[j];
+            next_vbase = a.vbase ? a.vbase[j] : a.scan[next_voxel];
         }
+    };
+    fetch(bat);
+    for (; bat < bat_end; bat += bat_step) {
+        const uint64_t i = (uint64_t)bat * 64u + lane;
+        const uint32_t voxel = next_voxel;
+        uint32_t cube = 0, nv = 0, vbase = next_vbase;
+        fetch(bat + bat_step);
+        if (i < active) {
+            const uint3 gp = calc_grid_pos(voxel, a.g.shift, a.g.mask);
+            float3 p;
+            p.x = ((float)gp.x - a.g.center.x) * a.g.vs.x;
+            p.y = ((float)gp.y - a.g.center.y) * a.g.vs.y;
+            p.z = ((float)gp.z - a.g.center.z) * a.g.vs.z;
+            const float3 vs = a.g.vs;
+            float3 v[8];
+            v[0] = p;
+            v[1] = make_float3(p.x + vs.x, p.y + 0.f, p.z + 0.f);
+            v[2] = make_float3(p.x + vs.x, p.y + vs.y, p.z + 0.f);
+            v[3] = make_float3(p.x + 0.f, p.y + vs.y, p.z + 0.f);
+            v[4] = make_float3(p.x + 0.f, p.y + 0.f, p.z + vs.z);
+            v[5] = make_float3(p.x + vs.x, p.y + 0.f, p.z + vs.z);
+            v[6] = make_float3(p.x + vs.x, p.y + vs.y, p.z + vs.z);
+            v[7] = make_float3(p.x + 0.f, p.y + vs.y, p.z + vs.z);
+            const uint32_t gx = a.g.size.x, gxy = a.g.size.x * a.g.size.y;
+            const uint32_t lin = gp.z * gxy + gp.y * gx + gp.x;
+            if (a.bits) {
+                cube = cube_from_bits(a.bits, lin, gx, gxy);
+            } else {
+                uint32_t idx[8];
+                corner_index(gp, a.g.size, idx);
+                cube = cube_index(a.flags, a.v, idx, a.iso, a.lo, a.hi, nf, nd, nl);
+            }
+            nv = a.nverts[cube];
+            // the corner values each mode's interpolation reads (corner order of
+            // the reference; grid_points whole, for their edge parameters)
+            const uint32_t f = a.flags;
+            const bool fixed = f & GC_BOOL_FIXED, make = f & GC_BOOL_MAKE_REGION;
+            float D[8], L[8];
+            float4 G[8];
+            const bool load = nv != 0;  // (a mode's volumes exist when it yields vertices)
+            if (load && !make && (fixed || !(f & GC_BOOL_DYNAMIC)))
+                corners_inside(a.v.dyn, lin, gx, gxy, D);
+            if (load && !make && (fixed || (f & GC_BOOL_DYNAMIC)))
+                corners_inside(a.v.lat, lin, gx, gxy, L);
+            if (load && (make || !fixed)) {
+                const uint32_t cl[8] = {lin, lin + 1, lin + gx + 1, lin + gx,
+                                        lin + gxy, lin + gxy + 1, lin + gxy + gx + 1, lin + gxy + gx};
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (k != 6) G[k] = ((const float4 *)a.v.fixed)[cl[k]];  // (corner 6: no edge)
+            }
+            // edge parameter of corner c along axis ax (t_x, t_y, t_z)
+            auto te = [&](int c, int ax) { return ax == 0 ? G[c].y : (ax == 1 ? G[c].z : G[c].w); };
+            if (nv == 0) {
+                // nothing to write (src/MarchingCubes_kernel.cu:2078)
+            } else if (make) {
+#pragma unroll
+                for (int e = 0; e < 12; ++e)
+                    vl[e][lane] = lerp3(v[c_pa[e]], v[c_pb[e]], te(c_pa[e], c_pax[e]));
+            } else if (fixed) {
+#pragma unroll
+                for (int e = 0; e < 12; ++e) {
+                    const int ca = c_na[e], cb = c_nb[e];
+                    vl[e][lane] = interp_new(a.iso, a.lo, a.hi, v[ca], v[cb], D[ca], D[cb], L[ca],
+                                             L[cb]);
+                }
+            } else if (f & GC_BOOL_DYNAMIC) {
+#pragma unroll
+                for (int e = 0; e < 12; ++e) {
+                    const int ca = c_pa[e], cb = c_pb[e];
+                    vl[e][lane] = interp_primitive_one(a.lo, a.hi, v[ca], v[cb], L[ca], L[cb],
+                                                       te(ca, c_pax[e]));
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 12; ++e) {
+                    const int ca = c_pa[e], cb = c_pb[e];
+                    vl[e][lane] = interp_primitive(a.iso, v[ca], v[cb], D[ca], D[cb],
+                                                   te(ca, c_pax[e]));
+                }
+            }
+        }
+        const uint32_t incl = wave_inclusive_scan(nv);
+        const uint32_t first = incl - nv;
+        const uint32_t total = __shfl(incl, 63, 64);
+        s_first[lane] = first;
+        s_base[lane] = vbase;
+        s_tri[lane] = *(const uint4 *)(a.tri + cube * 16);
+        for (uint32_t j = 0; j < nv; ++j) owner[first + j] = (uint8_t)lane;
+        __syncthreads();
+        for (uint32_t k = lane; k < total; k += 64) {
+            const uint32_t o = owner[k];
+            const uint32_t j = k - s_first[o];
+            const uint32_t t0 = j - j % 3u;  // the triangle's first vertex
+            const uint8_t *row = (const uint8_t *)&s_tri[o];
+            const float3 a0 = vl[row[t0]][o];
+            const float3 a1 = vl[row[t0 + 1]][o];
+            const float3 a2 = vl[row[t0 + 2]][o];
+            // calcNormal(ver[0], ver[2], ver[1]) = cross(v2 - v0, v1 - v0)
+            const float3 e0 = make_float3(a2.x - a0.x, a2.y - a0.y, a2.z - a0.z);
+            const float3 e1 = make_float3(a1.x - a0.x, a1.y - a0.y, a1.z - a0.z);
+            const float3 n = cross3(e0, e1);
+            const uint32_t index = s_base[o] + t0;  // scan[voxel] + t0, uint arithmetic
+            if (index < a.max_verts - 3u) {
+                const uint32_t r = j - t0;
+                const float3 q = r == 0 ? a0 : (r == 1 ? a1 : a2);
+                __builtin_nontemporal_store(F4nt{q.x, q.y, q.z, 1.0f},
+                                            (F4nt *)(a.pos + index + r));
+                __builtin_nontemporal_store(F4nt{n.x, n.y, n.z, 0.5f},
+                                            (F4nt *)(a.norm + index + r));
+            }
+        }
+        __syncthreads();  // the next batch reuses the LDS
     }
 }
 
@@ -532,28 +630,31 @@ int gpucad_compute_isosurface(gc_float4 *pos, gc_float4 *norm, float iso, uint32
            "NULL buffer");
     const Vols v = {(const GridPoint *)primitive_fixed, primitive_dynamic, lattice_field};
     const uint32_t n = num_voxels;
-    int r = bool_classify(d_voxel_verts, d_voxel_occupied, v, grid_size, grid_size_shift,
-                          grid_size_mask, n, iso, iso1, iso2, flags, 0);
+    GC_ARG(d_voxel_verts && d_voxel_occupied, "NULL buffer");
+    int r = check_dropin_grid(grid_size, grid_size_shift, grid_size_mask, n);
     if (r) return r;
-    // the occupancy scan also writes the compacted voxel list (nothing when
-    // no voxel is active, as the reference's early return leaves it)
-    if ((r = dropin_scan_u32(d_voxel_occupied_scan, d_voxel_occupied, n, 0, d_comp_voxel_array)))
+    if ((r = check_vols(flags, v.fixed, v.dyn, v.lat))) return r;
+    const DeviceTables *t = nullptr;
+    uint32_t *hw = nullptr;
+    if ((r = device_tables(&t)) || (r = host_words(&hw))) return r;
+    // classify (src/Isosurface.cu:63-67) from the point occupancy bits, both
+    // scans and the compaction (69-116); the totals stay on the device, so
+    // the call synchronises once
+    DropinScratch w = {};
+    if ((r = dropin_scratch(grid_size, n, &w))) return r;
+    const uint32_t np = grid_size.x * grid_size.y * grid_size.z;
+    hipLaunchKernelGGL(bool_point_bits_kernel, dim3(w.bits_blocks), dim3(256), 0, 0, w.bits, v, np,
+                       iso, iso1, iso2, flags);
+    GC_CHECK_LAUNCH();
+    if ((r = dropin_classify_scans(w, d_voxel_verts, d_voxel_occupied, d_voxel_occupied_scan,
+                                   d_voxel_verts_scan, d_comp_voxel_array, grid_size,
+                                   grid_size_shift, grid_size_mask, n, hw, 0)))
         return r;
-    uint32_t last[2] = {0, 0};
-    if ((r = dropin_read_last_pair(d_voxel_occupied, d_voxel_occupied_scan, n, last, 0)))
-        return r;
-    *active_voxels = last[0] + last[1];
-    if (*active_voxels == 0) {  // src/Isosurface.cu:83-87 (nfacets left as it was)
-        *total_verts = 0;
-        return GPUCAD_OK;
-    }
-    if ((r = dropin_scan_u32(d_voxel_verts_scan, d_voxel_verts, n, 0))) return r;
-    if ((r = dropin_read_last_pair(d_voxel_verts, d_voxel_verts_scan, n, last, 0))) return r;
-    *total_verts = last[0] + last[1];
-    *nfacets = *total_verts / 3;
-    // src/Isosurface.cu:120-121 clears max_verts bytes (not elements) of each
-    GC_HIP(hipMemset(pos, 0, max_verts));
-    GC_HIP(hipMemset(norm, 0, max_verts));
+    // src/Isosurface.cu:120-121 clears max_verts bytes (not elements) of each,
+    // when some voxel is active
+    hipLaunchKernelGGL(clear_if_active_kernel, dim3(1024), dim3(256), 0, 0, (uint8_t *)pos,
+                       (uint8_t *)norm, (uint64_t)max_verts, w.tot);
+    GC_CHECK_LAUNCH();
     BGenArgs a;
     a.pos = (float4 *)pos;
     a.norm = (float4 *)norm;
@@ -564,11 +665,24 @@ int gpucad_compute_isosurface(gc_float4 *pos, gc_float4 *norm, float iso, uint32
     a.iso = iso;
     a.lo = iso1;
     a.hi = iso2;
-    a.active = *active_voxels;
+    a.active = 0;
     a.max_verts = max_verts;
     a.flags = flags;
-    if ((r = bool_generate(a, 0))) return r;
+    a.tri = t->tri_table;
+    a.nverts = t->num_verts;
+    a.active_dev = w.tot;
+    a.vbase = w.vbase;
+    a.bits = (const uint32_t *)w.bits;
+    hipLaunchKernelGGL(bool_generate_kernel, dim3(dropin_generate_blocks(n)), dim3(64), 0, 0, a);
+    GC_CHECK_LAUNCH();
     GC_HIP(hipStreamSynchronize(0));
+    *active_voxels = hw[0];
+    if (*active_voxels == 0) {  // src/Isosurface.cu:83-87 (nfacets left as it was)
+        *total_verts = 0;
+        return GPUCAD_OK;
+    }
+    *total_verts = hw[1];
+    *nfacets = *total_verts / 3;
     return GPUCAD_OK;
 }
 

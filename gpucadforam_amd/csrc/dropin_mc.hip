@@ -15,12 +15,14 @@
 
 // Fixed scratch of gpucad_compute_isosurface_lattice_noalloc: the tile sums
 // of its two scans (occupancy, then vertex counts), kNoAllocMaxTiles words
-// each, 2 MiB per device, held until the process ends.  Host code never uses
-// the array's name as an address: noalloc_setup resolves the array's address
-// on each device and the entry point passes that pointer to the kernels.
+// each, then the two totals: 2 MiB + 256 B per device, held until the
+// process ends.  Host code never uses the array's name as an address:
+// noalloc_setup resolves the array's address on each device and the entry
+// point passes that pointer to the kernels.
 constexpr uint32_t kNoAllocMaxTiles = 1u << 18;
 constexpr uint32_t kNoAllocMaxVoxels = 1u << 30;
-static __device__ __align__(256) uint32_t scratch[2 * kNoAllocMaxTiles];
+constexpr uint32_t kNoAllocFixedWords = 2 * kNoAllocMaxTiles + 64;
+static __device__ __align__(256) uint32_t scratch[kNoAllocFixedWords];
 
 namespace gc {
 namespace {
@@ -80,6 +82,10 @@ __global__ void classify_kernel(uint32_t *verts, uint32_t *occ, const float *vol
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
+// point bits: 64-bit words per wave, points per wave, per 256-thread block
+constexpr int kBitsWaveWords = kDropinBitsWaveWords;
+constexpr uint32_t kBitsWavePoints = 64 * kBitsWaveWords;
+constexpr uint32_t kBitsBlockPoints = 4 * kBitsWavePoints;
 static_assert((uint64_t)kNoAllocMaxTiles * kScanTile == kNoAllocMaxVoxels,
               "the fixed scratch holds the tile sums of the largest allocation-free grid");
 
@@ -215,6 +221,380 @@ __global__ void classify_tile_kernel(uint32_t *verts, uint32_t *occ, const float
     }
 }
 
+// ---- the orchestrations' classify and scans from point occupancy bits
+//
+// Bit p of `bits` (uint32 words, bit p % 32 of word p / 32) is corner p's
+// side of the surface.  Corner reads in the reference clamp to gridSize, and
+// check_grid only admits layouts whose cells keep all eight corners inside
+// the volume, so a cell's corners are the points lin, lin + 1, lin + gx,
+// lin + gx + 1 and the same in the next plane (lin = z*gx*gy + y*gx + x).
+
+// bit p of bits = vol[p] < iso (the cube-index test of classifyVoxel_lattice_new);
+// each wave sets kBitsWavePoints consecutive bits
+__global__ void __launch_bounds__(256) point_bits_kernel(uint64_t *bits, const float *vol,
+                                                         uint32_t np, float iso) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint64_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < kBitsWaveWords; ++j) {
+        const uint64_t i = wave * kBitsWavePoints + j * 64 + lane;
+        const uint64_t m = __ballot(i < np && vol[i] < iso);
+        if (lane == (uint32_t)j) mine = m;
+    }
+    if (lane < kBitsWaveWords) bits[wave * kBitsWaveWords + lane] = mine;
+}
+
+// The cells of a scan tile in groups of 4 consecutive cells: thread t's
+// groups start at cells tile + k * kGroupStep + 4t (k < kGroups), each
+// kGroupStep cells on from the last, walked without a division per group.
+constexpr int kGroups = kScanItems / 4;
+constexpr uint32_t kGroupStep = kScanThreads * 4;
+
+struct CellWalk {
+    uint3 shift, mask;       // calcGridPos of the first group
+    uint32_t gx, gxy;        // point strides: gridSize.x, gridSize.x * gridSize.y
+    uint32_t dx, dy, dz;     // kGroupStep cells as steps along x, y, z
+};
+
+CellWalk cell_walk(gc_uint3 s, gc_uint3 sh, gc_uint3 m) {
+    CellWalk w;
+    w.shift = make_uint3(sh.x, sh.y, sh.z);
+    w.mask = make_uint3(m.x, m.y, m.z);
+    w.gx = s.x;
+    w.gxy = s.x * s.y;
+    const uint32_t plane = m.x * m.y;  // == gridSizeShift.z (check_grid)
+    w.dz = kGroupStep / plane;
+    w.dy = (kGroupStep % plane) / m.x;
+    w.dx = (kGroupStep % plane) % m.x;
+    return w;
+}
+
+__device__ __forceinline__ void walk(uint3 &p, uint32_t dx, uint32_t dy, uint32_t dz,
+                                     const uint3 &mask) {
+    // dx < mask.x and dy < mask.y, so one carry per axis suffices
+    p.x += dx;
+    p.y += dy;
+    p.z += dz;
+    if (p.x >= mask.x) {
+        p.x -= mask.x;
+        ++p.y;
+    }
+    if (p.y >= mask.y) {
+        p.y -= mask.y;
+        ++p.z;
+    }
+}
+
+// 32 bits of the point bits from point p on
+__device__ __forceinline__ uint32_t bit_window(const uint32_t *w, uint32_t p) {
+    const uint32_t k = p >> 5;
+    return __builtin_amdgcn_alignbit(w[k + 1], w[k], p & 31u);
+}
+
+// vertex counts of the group of cells c0 .. c0+3 (those at or beyond n
+// count 0) whose first cell is p; c0 < n.  Four 32-bit windows of the bits
+// from corner 0's point of the first cell, one per corner row, hold the
+// corners of all four cells unless the group crosses a plane (cells past
+// the end of the row are gridSize.x - gridSizeMask.x points further on;
+// rows of at least 4 cells, so a group crosses at most one row end).
+__device__ __forceinline__ void group_counts(uint32_t (&nv)[4], const uint32_t *bits, uint3 p,
+                                             uint64_t c0, uint32_t n, const CellWalk &cw,
+                                             const uint8_t *s_nv) {
+    const uint32_t skip = cw.gx - cw.mask.x;
+    if (p.x + 3 < cw.mask.x || (p.y + 1 < cw.mask.y && skip <= 27 && cw.mask.x >= 4)) {
+        const uint32_t lin = p.z * cw.gxy + p.y * cw.gx + p.x;
+        const uint32_t w0 = bit_window(bits, lin), w1 = bit_window(bits, lin + cw.gx);
+        const uint32_t w2 = bit_window(bits, lin + cw.gxy);
+        const uint32_t w3 = bit_window(bits, lin + cw.gxy + cw.gx);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const uint32_t sh = c + (p.x + c >= cw.mask.x ? skip : 0u);
+            const uint32_t cube = ((w0 >> sh) & 3u) | swap2((w1 >> sh) & 3u) << 2 |
+                                  ((w2 >> sh) & 3u) << 4 | swap2((w3 >> sh) & 3u) << 6;
+            nv[c] = c0 + c < n ? s_nv[cube] : 0u;
+        }
+    } else {  // the group crosses a plane (or rows are far apart)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            nv[c] = 0;
+            if (c0 + c < n) {
+                const uint32_t lin = p.z * cw.gxy + p.y * cw.gx + p.x;
+                nv[c] = s_nv[cube_from_bits(bits, lin, cw.gx, cw.gxy)];
+                walk(p, 1, 0, 0, cw.mask);
+            }
+        }
+    }
+}
+
+// the scan's first pass from the point bits: the tile's sums of vertex
+// counts and occupancy
+__global__ void __launch_bounds__(kScanThreads) tile_sums_bits_kernel(
+    const uint32_t *bits, CellWalk cw, uint32_t n, const uint8_t *nverts, uint32_t *bsum_v,
+    uint32_t *bsum_o) {
+    constexpr int kWaves = kScanThreads / 64;
+    static_assert(kScanThreads == 256, "one table entry per thread");
+    __shared__ uint8_t s_nv[256];
+    __shared__ uint32_t part[2][kWaves];
+    s_nv[threadIdx.x] = nverts[threadIdx.x];
+    __syncthreads();
+    const uint64_t g0 = (uint64_t)blockIdx.x * kScanTile + 4u * threadIdx.x;
+    uint32_t sv = 0, so = 0;
+    if (g0 < n) {
+        uint3 p = calc_grid_pos((uint32_t)g0, cw.shift, cw.mask);
+#pragma unroll
+        for (int k = 0; k < kGroups; ++k) {
+            const uint64_t c0 = g0 + (uint64_t)k * kGroupStep;
+            if (c0 < n) {
+                uint32_t nv[4];
+                group_counts(nv, bits, p, c0, n, cw, s_nv);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    sv += nv[c];
+                    so += nv[c] > 0;
+                }
+            }
+            walk(p, cw.dx, cw.dy, cw.dz, cw.mask);
+        }
+    }
+    sv = wave_sum(sv);
+    so = wave_sum(so);
+    if ((threadIdx.x & 63) == 0) {
+        part[0][threadIdx.x >> 6] = sv;
+        part[1][threadIdx.x >> 6] = so;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tv = 0, to = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            tv += part[0][w];
+            to += part[1][w];
+        }
+        bsum_v[blockIdx.x] = tv;
+        bsum_o[blockIdx.x] = to;
+    }
+}
+
+typedef uint32_t U4nt __attribute__((ext_vector_type(4)));
+
+// the outputs of the orchestrations' classify, scans and compaction
+struct ScanOut {
+    uint32_t *verts, *occ, *occ_scan, *verts_scan, *comp, *vbase;
+    int vec;  // verts, occ, occ_scan and verts_scan are 16-byte aligned
+};
+
+__device__ __forceinline__ void store4(uint32_t *p, uint64_t c0, const uint32_t (&v)[4], bool vec,
+                                       uint32_t n) {
+    if (vec && c0 + 3 < n) {
+        __builtin_nontemporal_store(U4nt{v[0], v[1], v[2], v[3]}, (U4nt *)(p + c0));
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (c0 + c < n) p[c0 + c] = v[c];
+    }
+}
+
+// the scans' third pass from the point bits: every array the reference's
+// classify, scans and compaction write (verts_scan only when some voxel is
+// active), each tile's cells classified again from the bits (in L2) rather
+// than read back; vbase[j] = verts_scan of the j-th active cell
+__global__ void __launch_bounds__(kScanThreads) classify_scan_bits_kernel(
+    ScanOut o, const uint32_t *bits, CellWalk cw, uint32_t n, const uint8_t *nverts,
+    const uint32_t *bsum_o, const uint32_t *bsum_v, const uint32_t *tot) {
+    constexpr int kWaves = kScanThreads / 64;
+    constexpr int kChunks = kGroups * kWaves;
+    static_assert(kScanThreads == 256 && kChunks <= 64, "tile layout");
+    __shared__ uint8_t s_nv[256];
+    __shared__ uint32_t wtot[2][kChunks];
+    s_nv[threadIdx.x] = nverts[threadIdx.x];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t g0 = (uint64_t)blockIdx.x * kScanTile + 4u * threadIdx.x;
+    uint32_t nv[kGroups][4];
+#pragma unroll
+    for (int k = 0; k < kGroups; ++k)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) nv[k][c] = 0;
+    if (g0 < n) {
+        uint3 p = calc_grid_pos((uint32_t)g0, cw.shift, cw.mask);
+#pragma unroll
+        for (int k = 0; k < kGroups; ++k) {
+            const uint64_t c0 = g0 + (uint64_t)k * kGroupStep;
+            if (c0 < n) group_counts(nv[k], bits, p, c0, n, cw, s_nv);
+            walk(p, cw.dx, cw.dy, cw.dz, cw.mask);
+        }
+    }
+    // chunk (k, wave) holds 256 consecutive cells, 4 per lane
+    uint32_t ex_o[kGroups], ex_v[kGroups];
+#pragma unroll
+    for (int k = 0; k < kGroups; ++k) {
+        uint32_t so = 0, sv = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            so += nv[k][c] > 0;
+            sv += nv[k][c];
+        }
+        const uint32_t io = wave_inclusive_scan(so), iv = wave_inclusive_scan(sv);
+        ex_o[k] = io - so;
+        ex_v[k] = iv - sv;
+        if (lane == 63) {
+            wtot[0][k * kWaves + wave] = io;
+            wtot[1][k * kWaves + wave] = iv;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const uint32_t vo = lane < kChunks ? wtot[0][lane] : 0u;
+        const uint32_t vv = lane < kChunks ? wtot[1][lane] : 0u;
+        const uint32_t eo = wave_inclusive_scan(vo) - vo + bsum_o[blockIdx.x];
+        const uint32_t ev = wave_inclusive_scan(vv) - vv + bsum_v[blockIdx.x];
+        if (lane < kChunks) {
+            wtot[0][lane] = eo;
+            wtot[1][lane] = ev;
+        }
+    }
+    __syncthreads();
+    const bool any_active = tot[0] != 0;
+    const bool vec = o.vec != 0;
+#pragma unroll
+    for (int k = 0; k < kGroups; ++k) {
+        const uint64_t c0 = g0 + (uint64_t)k * kGroupStep;
+        if (c0 >= n) continue;
+        uint32_t eo = wtot[0][k * kWaves + wave] + ex_o[k];
+        uint32_t ev = wtot[1][k * kWaves + wave] + ex_v[k];
+        uint32_t occ[4], so[4], sv[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            occ[c] = nv[k][c] > 0;
+            so[c] = eo;
+            sv[c] = ev;
+            if (occ[c] && c0 + c < n) {
+                o.comp[eo] = (uint32_t)(c0 + c);
+                if (o.vbase) o.vbase[eo] = ev;
+            }
+            eo += occ[c];
+            ev += nv[k][c];
+        }
+        store4(o.verts, c0, nv[k], vec, n);
+        store4(o.occ, c0, occ, vec, n);
+        store4(o.occ_scan, c0, so, vec, n);
+        if (any_active) store4(o.verts_scan, c0, sv, vec, n);
+    }
+}
+
+// both scans' second pass (one block of kSumsThreads, each scanning a run
+// of consecutive tile sums of both arrays): the totals (active voxels,
+// vertices, wrapping as the reference's last element + last scan element)
+// go to tot[0..1] and to the host words
+constexpr int kSumsThreads = 1024;
+__global__ void __launch_bounds__(kSumsThreads) scan_sums2_kernel(uint32_t *bsum_o,
+                                                                  uint32_t *bsum_v, uint32_t nb,
+                                                                  uint32_t *tot, uint32_t *hw) {
+    constexpr int kWaves = kSumsThreads / 64;
+    __shared__ uint32_t s_o[kWaves + 1], s_v[kWaves + 1];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t per = (nb + kSumsThreads - 1) / kSumsThreads;
+    const uint32_t beg = min(nb, threadIdx.x * per), end = min(nb, beg + per);
+    uint32_t ao = 0, av = 0;
+    for (uint32_t i = beg; i < end; ++i) {
+        ao += bsum_o[i];
+        av += bsum_v[i];
+    }
+    const uint32_t io = wave_inclusive_scan(ao), iv = wave_inclusive_scan(av);
+    if (lane == 63) {
+        s_o[threadIdx.x >> 6] = io;
+        s_v[threadIdx.x >> 6] = iv;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const uint32_t vo = lane < kWaves ? s_o[lane] : 0u;
+        const uint32_t vv = lane < kWaves ? s_v[lane] : 0u;
+        const uint32_t jo = wave_inclusive_scan(vo), jv = wave_inclusive_scan(vv);
+        if (lane < kWaves) {
+            s_o[lane] = jo - vo;
+            s_v[lane] = jv - vv;
+        }
+        if (lane == kWaves - 1) {
+            s_o[kWaves] = jo;
+            s_v[kWaves] = jv;
+        }
+    }
+    __syncthreads();
+    uint32_t ro = s_o[threadIdx.x >> 6] + io - ao, rv = s_v[threadIdx.x >> 6] + iv - av;
+    for (uint32_t i = beg; i < end; ++i) {
+        const uint32_t vo = bsum_o[i], vv = bsum_v[i];
+        bsum_o[i] = ro;
+        bsum_v[i] = rv;
+        ro += vo;
+        rv += vv;
+    }
+    if (threadIdx.x == 0) {
+        tot[0] = s_o[kWaves];
+        tot[1] = s_v[kWaves];
+        hw[0] = s_o[kWaves];
+        hw[1] = s_v[kWaves];
+    }
+}
+
+// exclusive scan of a tile's kScanItems x kScanThreads values (element
+// k * kScanThreads + threadIdx.x is vals[k]) given the tile's offset
+__device__ __forceinline__ void tile_scan(const uint32_t (&vals)[kScanItems],
+                                          uint32_t (&excl)[kScanItems], uint32_t offset,
+                                          uint32_t *wtot) {
+    constexpr int kWaves = kScanThreads / 64;
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        excl[k] = wave_inclusive_scan(vals[k]);
+        if ((threadIdx.x & 63) == 63) wtot[k * kWaves + wave] = excl[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < kScanItems * kWaves) {
+        const uint32_t v = wtot[threadIdx.x];
+        wtot[threadIdx.x] = wave_inclusive_scan(v) - v + offset;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) excl[k] += wtot[k * kWaves + wave] - vals[k];
+}
+
+// both scans' third pass from verts alone (occ[i] = verts[i] > 0, as the
+// classify wrote it): occ_scan, the compacted list (with, when vbase is set,
+// each active voxel's verts_scan value beside it) and verts_scan, except that
+// with no active voxel verts_scan is not written (the reference returns
+// before that scan)
+__global__ void __launch_bounds__(kScanThreads) scan_apply2_kernel(
+    uint32_t *occ_scan, uint32_t *comp, uint32_t *verts_scan, uint32_t *vbase,
+    const uint32_t *verts, uint32_t n, const uint32_t *bsum_o, const uint32_t *bsum_v,
+    const uint32_t *tot) {
+    constexpr int kWaves = kScanThreads / 64;
+    __shared__ uint32_t wtot[2][kScanItems * kWaves];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x;
+    uint32_t vals[kScanItems], occ[kScanItems], eo[kScanItems], ev[kScanItems];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + k * kScanThreads;
+        vals[k] = i < n ? verts[i] : 0u;
+        occ[k] = vals[k] != 0;
+    }
+    tile_scan(occ, eo, bsum_o[blockIdx.x], wtot[0]);
+    tile_scan(vals, ev, bsum_v[blockIdx.x], wtot[1]);
+    const bool any_active = tot[0] != 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + k * kScanThreads;
+        if (i < n) {
+            occ_scan[i] = eo[k];
+            if (any_active) verts_scan[i] = ev[k];
+            if (occ[k]) {
+                comp[eo[k]] = (uint32_t)i;
+                if (vbase) vbase[eo[k]] = ev[k];
+            }
+        }
+    }
+}
+
 __global__ void compact_kernel(uint32_t *comp, const uint32_t *occ, const uint32_t *occ_scan,
                                uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -312,9 +692,13 @@ struct GenArgs {
     GridDesc g;
     float iso, lo1, hi1, lo2, hi2;
     uint32_t active, max_verts;
+    const uint32_t *active_dev = nullptr;  // when set, the active voxel count is read here
+    const uint32_t *vbase = nullptr;       // when set, scan[comp[i]] is read as vbase[i]
     const uint8_t *tri;  // 256*16
     const uint8_t *nverts;
 };
+
+typedef float F4nt __attribute__((ext_vector_type(4)));
 
 // One wave of 64 active voxels (one per lane): each lane interpolates its
 // 12 edge vertices into LDS as the reference's thread does; then the wave
@@ -323,92 +707,129 @@ struct GenArgs {
 // only if its first index is below maxVerts - 3, as in the reference; with
 // the scan of the voxels' vertex counts these are one contiguous range
 // (active voxels are compacted in voxel order), so every store of pos and
-// norm is coalesced.
+// norm is coalesced (and non-temporal: nothing reads them back here).
+//
+// Batches of 64 active voxels: a launch of one workgroup per batch takes
+// batch blockIdx.x; the orchestrations' persistent launch (active count on
+// the device, gridDim.x a multiple of 8) gives each XCD (workgroup b runs on
+// XCD b % 8) a contiguous eighth of the batches, so the corner gathers of
+// its waves share its L2.  Field two is read only by a wave with a corner
+// whose indicator is 2 (no other corner takes vertexInterp3_new's second
+// branch).
 template <int MODE>
 __global__ void __launch_bounds__(64) generate_kernel(GenArgs a) {
-    __shared__ float3 vl[12][64];
+    __shared__ float3 vl[12][65];  // (stride 65: the edges of one voxel in different banks)
     __shared__ uint8_t owner[64 * 15];  // output vertex -> lane
     __shared__ uint32_t s_first[64], s_base[64];
     __shared__ uint4 s_tri[64];  // each lane's triTable row
     const uint32_t lane = threadIdx.x;
-    const uint32_t i = blockIdx.x * 64u + lane;
-    uint32_t cube = 0, nv = 0, vbase = 0;
-    if (i < a.active) {
-        const uint32_t voxel = a.comp[i];
-        vbase = a.scan[voxel];
-        const uint3 gp = calc_grid_pos(voxel, a.g.shift, a.g.mask);
-        float3 p;
-        p.x = ((float)gp.x - a.g.center.x) * a.g.vs.x;
-        p.y = ((float)gp.y - a.g.center.y) * a.g.vs.y;
-        p.z = ((float)gp.z - a.g.center.z) * a.g.vs.z;
-        const float3 vs = a.g.vs;
-        float3 v[8];
-        v[0] = p;
-        v[1] = make_float3(p.x + vs.x, p.y + 0.f, p.z + 0.f);
-        v[2] = make_float3(p.x + vs.x, p.y + vs.y, p.z + 0.f);
-        v[3] = make_float3(p.x + 0.f, p.y + vs.y, p.z + 0.f);
-        v[4] = make_float3(p.x + 0.f, p.y + 0.f, p.z + vs.z);
-        v[5] = make_float3(p.x + vs.x, p.y + 0.f, p.z + vs.z);
-        v[6] = make_float3(p.x + vs.x, p.y + vs.y, p.z + vs.z);
-        v[7] = make_float3(p.x + 0.f, p.y + vs.y, p.z + vs.z);
-        float fid[8];
-        corners(a.vol, gp, a.g.size, fid);
-        cube = cube_of(fid, a.iso);
-        nv = a.nverts[cube];
-        if (MODE == GEN_STD) {
-            // an active voxel has nv > 0 (src/MarchingCubes_kernel.cu:3103)
+    const uint32_t active = a.active_dev ? *a.active_dev : a.active;
+    const uint32_t nbat = (uint32_t)(((uint64_t)active + 63) / 64);
+    uint32_t bat = blockIdx.x, bat_end = nbat, bat_step = gridDim.x;
+    if (a.active_dev) {
+        const uint32_t per = (nbat + 7) / 8, x = blockIdx.x & 7u;
+        bat = x * per + (blockIdx.x >> 3);
+        bat_end = min(nbat, x * per + per);
+        bat_step = gridDim.x >> 3;
+    }
+    // each batch's voxel ids and vertex offsets are read one batch ahead
+    uint32_t next_voxel = 0, next_vbase = 0;
+    auto fetch = [&](uint32_t b) {
+        const uint64_t j = (uint64_t)b * 64u + lane;
+        if (b < bat_end && j < active) {
+            next_voxel = a.comp[j];
+            next_vbase = a.vbase ? a.vbase[j] : a.scan[next_voxel];
+        }
+    };
+    fetch(bat);
+    for (; bat < bat_end; bat += bat_step) {
+        const uint64_t i = (uint64_t)bat * 64u + lane;
+        const uint32_t voxel = next_voxel;
+        uint32_t cube = 0, nv = 0, vbase = next_vbase;
+        fetch(bat + bat_step);
+        if (i < active) {
+            const uint3 gp = calc_grid_pos(voxel, a.g.shift, a.g.mask);
+            float3 p;
+            p.x = ((float)gp.x - a.g.center.x) * a.g.vs.x;
+            p.y = ((float)gp.y - a.g.center.y) * a.g.vs.y;
+            p.z = ((float)gp.z - a.g.center.z) * a.g.vs.z;
+            const float3 vs = a.g.vs;
+            float3 v[8];
+            v[0] = p;
+            v[1] = make_float3(p.x + vs.x, p.y + 0.f, p.z + 0.f);
+            v[2] = make_float3(p.x + vs.x, p.y + vs.y, p.z + 0.f);
+            v[3] = make_float3(p.x + 0.f, p.y + vs.y, p.z + 0.f);
+            v[4] = make_float3(p.x + 0.f, p.y + 0.f, p.z + vs.z);
+            v[5] = make_float3(p.x + vs.x, p.y + 0.f, p.z + vs.z);
+            v[6] = make_float3(p.x + vs.x, p.y + vs.y, p.z + vs.z);
+            v[7] = make_float3(p.x + 0.f, p.y + vs.y, p.z + vs.z);
+            const uint32_t gx = a.g.size.x, gxy = a.g.size.x * a.g.size.y;
+            const uint32_t lin = gp.z * gxy + gp.y * gx + gp.x;
+            float fid[8];
+            corners_inside(a.vol, lin, gx, gxy, fid);
+            cube = cube_of(fid, a.iso);
+            nv = a.nverts[cube];
+            if (MODE == GEN_STD) {
+                // an active voxel has nv > 0 (src/MarchingCubes_kernel.cu:3103)
 #pragma unroll
-            for (int e = 0; e < 12; ++e)
-                vl[e][lane] = interp4(a.iso, v[c_edge_a[e]], v[c_edge_b[e]], fid[c_edge_a[e]],
-                                      fid[c_edge_b[e]]);
-        } else {
-            float f1[8];
-            corners(a.vol1, gp, a.g.size, f1);
-            if (MODE == GEN_BAND2) {
-                float f2[8];
-                corners(a.vol2, gp, a.g.size, f2);
-#pragma unroll
-                for (int e = 0; e < 12; ++e) {
-                    int ca = c_edge_a[e], cb = c_edge_b[e];
-                    vl[e][lane] = interp3(a.lo1, a.hi1, v[ca], v[cb], f1[ca], f1[cb], f2[ca],
-                                          f2[cb], a.lo2, a.hi2, fid[ca], fid[cb]);
-                }
+                for (int e = 0; e < 12; ++e)
+                    vl[e][lane] = interp4(a.iso, v[c_edge_a[e]], v[c_edge_b[e]],
+                                          fid[c_edge_a[e]], fid[c_edge_b[e]]);
             } else {
+                float f1[8];
+                corners_inside(a.vol1, lin, gx, gxy, f1);
+                if (MODE == GEN_BAND2) {
+                    bool two = false;
 #pragma unroll
-                for (int e = 0; e < 12; ++e) {
-                    int ca = c_edge_a[e], cb = c_edge_b[e];
-                    vl[e][lane] =
-                        interp2(a.lo1, a.hi1, v[ca], v[cb], f1[ca], f1[cb], fid[ca], fid[cb]);
+                    for (int k = 0; k < 8; ++k) two |= fid[k] == 2.f;
+                    float f2[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    if (__any(two)) corners_inside(a.vol2, lin, gx, gxy, f2);
+#pragma unroll
+                    for (int e = 0; e < 12; ++e) {
+                        int ca = c_edge_a[e], cb = c_edge_b[e];
+                        vl[e][lane] = interp3(a.lo1, a.hi1, v[ca], v[cb], f1[ca], f1[cb], f2[ca],
+                                              f2[cb], a.lo2, a.hi2, fid[ca], fid[cb]);
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 12; ++e) {
+                        int ca = c_edge_a[e], cb = c_edge_b[e];
+                        vl[e][lane] =
+                            interp2(a.lo1, a.hi1, v[ca], v[cb], f1[ca], f1[cb], fid[ca], fid[cb]);
+                    }
                 }
             }
         }
-    }
-    const uint32_t incl = wave_inclusive_scan(nv);
-    const uint32_t first = incl - nv;
-    const uint32_t total = __shfl(incl, 63, 64);
-    s_first[lane] = first;
-    s_base[lane] = vbase;
-    s_tri[lane] = *(const uint4 *)(a.tri + cube * 16);
-    for (uint32_t j = 0; j < nv; ++j) owner[first + j] = (uint8_t)lane;
-    __syncthreads();
-    for (uint32_t k = lane; k < total; k += 64) {
-        const uint32_t o = owner[k];
-        const uint32_t j = k - s_first[o];
-        const uint32_t t0 = j - j % 3u;  // the triangle's first vertex
-        const uint8_t *row = (const uint8_t *)&s_tri[o];
-        const float3 a0 = vl[row[t0]][o];
-        const float3 a1 = vl[row[t0 + 1]][o];
-        const float3 a2 = vl[row[t0 + 2]][o];
-        const float3 e0 = make_float3(a1.x - a0.x, a1.y - a0.y, a1.z - a0.z);
-        const float3 e1 = make_float3(a2.x - a0.x, a2.y - a0.y, a2.z - a0.z);
-        const float3 n = cross3(e0, e1);
-        const uint32_t index = s_base[o] + t0;  // scan[voxel] + t0, uint arithmetic
-        if (index < a.max_verts - 3u) {
-            const uint32_t r = j - t0;
-            const float3 q = r == 0 ? a0 : (r == 1 ? a1 : a2);
-            a.pos[index + r] = make_float4(q.x, q.y, q.z, 1.0f);
-            a.norm[index + r] = make_float4(n.x, n.y, n.z, 0.0f);
+        const uint32_t incl = wave_inclusive_scan(nv);
+        const uint32_t first = incl - nv;
+        const uint32_t total = __shfl(incl, 63, 64);
+        s_first[lane] = first;
+        s_base[lane] = vbase;
+        s_tri[lane] = *(const uint4 *)(a.tri + cube * 16);
+        for (uint32_t j = 0; j < nv; ++j) owner[first + j] = (uint8_t)lane;
+        __syncthreads();
+        for (uint32_t k = lane; k < total; k += 64) {
+            const uint32_t o = owner[k];
+            const uint32_t j = k - s_first[o];
+            const uint32_t t0 = j - j % 3u;  // the triangle's first vertex
+            const uint8_t *row = (const uint8_t *)&s_tri[o];
+            const float3 a0 = vl[row[t0]][o];
+            const float3 a1 = vl[row[t0 + 1]][o];
+            const float3 a2 = vl[row[t0 + 2]][o];
+            const float3 e0 = make_float3(a1.x - a0.x, a1.y - a0.y, a1.z - a0.z);
+            const float3 e1 = make_float3(a2.x - a0.x, a2.y - a0.y, a2.z - a0.z);
+            const float3 n = cross3(e0, e1);
+            const uint32_t index = s_base[o] + t0;  // scan[voxel] + t0, uint arithmetic
+            if (index < a.max_verts - 3u) {
+                const uint32_t r = j - t0;
+                const float3 q = r == 0 ? a0 : (r == 1 ? a1 : a2);
+                __builtin_nontemporal_store(F4nt{q.x, q.y, q.z, 1.0f},
+                                            (F4nt *)(a.pos + index + r));
+                __builtin_nontemporal_store(F4nt{n.x, n.y, n.z, 0.0f},
+                                            (F4nt *)(a.norm + index + r));
+            }
         }
+        __syncthreads();  // the next batch reuses the LDS
     }
 }
 
@@ -529,10 +950,68 @@ int read_last_pair(const uint32_t *a, const uint32_t *b, uint32_t n, uint32_t *h
     return GPUCAD_OK;
 }
 
+// Both scans' second and third passes after a classify that left the tile
+// sums in w.bsum_o / w.bsum_v.  When a buffer reached only with active
+// voxels is NULL (later_null), the active count is read first and must be 0.
+int scans(const DropinScratch &w, const uint32_t *verts, uint32_t *occ_scan, uint32_t *verts_scan,
+          uint32_t *comp, uint32_t n, uint32_t *hw, bool later_null, hipStream_t st) {
+    hipLaunchKernelGGL(scan_sums2_kernel, dim3(1), dim3(kSumsThreads), 0, st, w.bsum_o, w.bsum_v,
+                       w.nb, w.tot, hw);
+    GC_CHECK_LAUNCH();
+    if (later_null) {
+        GC_HIP(hipStreamSynchronize(st));
+        GC_ARG(hw[0] == 0, "NULL buffer");
+    }
+    hipLaunchKernelGGL(scan_apply2_kernel, dim3(w.nb), dim3(kScanThreads), 0, st, occ_scan, comp,
+                       verts_scan, w.vbase, verts, n, w.bsum_o, w.bsum_v, w.tot);
+    GC_CHECK_LAUNCH();
+    return GPUCAD_OK;
+}
+
+int classify_scans(const DropinScratch &w, uint32_t *verts, uint32_t *occ, uint32_t *occ_scan,
+                   uint32_t *verts_scan, uint32_t *comp, gc_uint3 s, gc_uint3 sh, gc_uint3 m,
+                   uint32_t n, const DeviceTables *t, uint32_t *hw, bool later_null,
+                   hipStream_t st) {
+    const CellWalk cw = cell_walk(s, sh, m);
+    const uint32_t *bits = (const uint32_t *)w.bits;
+    hipLaunchKernelGGL(tile_sums_bits_kernel, dim3(w.nb), dim3(kScanThreads), 0, st, bits, cw, n,
+                       t->num_verts, w.bsum_v, w.bsum_o);
+    GC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(scan_sums2_kernel, dim3(1), dim3(kSumsThreads), 0, st, w.bsum_o, w.bsum_v,
+                       w.nb, w.tot, hw);
+    GC_CHECK_LAUNCH();
+    if (later_null) {
+        GC_HIP(hipStreamSynchronize(st));
+        GC_ARG(hw[0] == 0, "NULL buffer");
+    }
+    ScanOut o;
+    o.verts = verts;
+    o.occ = occ;
+    o.occ_scan = occ_scan;
+    o.verts_scan = verts_scan;
+    o.comp = comp;
+    o.vbase = w.vbase;
+    auto a16 = [](const void *q) { return ((uintptr_t)q & 15u) == 0; };
+    o.vec = a16(verts) && a16(occ) && a16(occ_scan) && a16(verts_scan);
+    hipLaunchKernelGGL(classify_scan_bits_kernel, dim3(w.nb), dim3(kScanThreads), 0, st, o, bits,
+                       cw, n, t->num_verts, (const uint32_t *)w.bsum_o, (const uint32_t *)w.bsum_v,
+                       (const uint32_t *)w.tot);
+    GC_CHECK_LAUNCH();
+    return GPUCAD_OK;
+}
+
 // Isosurface.cu:401-486 / 488-572 orchestration; MODE selects the generate.
-// With `fixed` (2 * kNoAllocMaxTiles words of device memory) the scans'
-// tile sums go there and the tables and host words must already exist:
-// nothing is allocated.
+// Classification goes through the point occupancy bits; both scans share
+// one pass over the tile sums and one over verts; the totals stay on the
+// device for the generate launch (a persistent grid that reads the active
+// count), so the call synchronises once.  Caller-visible buffers end as the
+// reference leaves them: with no active voxel, verts_scan, the compacted
+// list, pos and norm are not written.
+// With `fixed` (kNoAllocFixedWords words of device memory) nothing is
+// allocated: the tables and host words must already exist, cells are
+// classified from their corners directly and the generate gathers scan[].
+constexpr uint32_t kGenerateBlocks = 256 * 12;  // a multiple of the 8 XCDs
+
 template <int MODE>
 int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float iso, uint32_t n,
                        uint32_t *d_verts, uint32_t *d_verts_scan, uint32_t *d_occ,
@@ -543,6 +1022,10 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
     GC_ARG(active && total, "host outputs are NULL");
     GC_ARG(n > 0, "numVoxels must be > 0");
     GC_ARG(d_verts && d_occ && vol && d_occ_scan, "NULL buffer");
+    // buffers the reference reaches only when some voxel is active may be
+    // NULL; the call then learns the active count before going on
+    const bool later_null = !d_verts_scan || !d_comp || !pos || !norm ||
+                            (MODE != GEN_STD && !vol1) || (MODE == GEN_BAND2 && !vol2);
     hipStream_t st = 0;
     int r = check_grid(s, sh, m, n);
     if (r) return r;
@@ -550,34 +1033,28 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
     uint32_t *hw = nullptr;
     if ((r = fixed ? tables_if_ready(&t) : device_tables(&t))) return r;
     if ((r = fixed ? host_words_if_ready(&hw) : host_words(&hw))) return r;
-    // scratch: the tile sums of occ and verts
-    const uint32_t nb = (uint32_t)(((uint64_t)n + kScanTile - 1) / kScanTile);
-    uint32_t *bsum_o = fixed, *bsum_v = fixed + kNoAllocMaxTiles;
-    if (!fixed) {
-        void *ws = nullptr;
-        if ((r = scratch_arena(2 * (size_t)nb * sizeof(uint32_t), &ws))) return r;
-        bsum_o = (uint32_t *)ws;
-        bsum_v = bsum_o + nb;
+    DropinScratch w = {};
+    if (fixed) {
+        w.nb = (uint32_t)(((uint64_t)n + kScanTile - 1) / kScanTile);
+        w.bsum_o = fixed;
+        w.bsum_v = fixed + kNoAllocMaxTiles;
+        w.tot = fixed + 2 * kNoAllocMaxTiles;
+        // classify (src/Isosurface.cu:408-420) with the first pass of both scans
+        const GridDesc g = grid_desc(s, sh, m, gc_float3{1, 1, 1}, gc_float3{0, 0, 0});
+        hipLaunchKernelGGL(classify_tile_kernel, dim3(w.nb), dim3(kScanThreads), 0, st, d_verts,
+                           d_occ, vol, g, n, iso, t->num_verts, w.bsum_v, w.bsum_o);
+        GC_CHECK_LAUNCH();
+        if ((r = scans(w, d_verts, d_occ_scan, d_verts_scan, d_comp, n, hw, later_null, st)))
+            return r;
+    } else {
+        if ((r = dropin_scratch(s, n, &w))) return r;
+        hipLaunchKernelGGL(point_bits_kernel, dim3(w.bits_blocks), dim3(256), 0, st, w.bits, vol,
+                           s.x * s.y * s.z, iso);
+        GC_CHECK_LAUNCH();
+        if ((r = classify_scans(w, d_verts, d_occ, d_occ_scan, d_verts_scan, d_comp, s, sh, m, n,
+                                t, hw, later_null, st)))
+            return r;
     }
-    // classify (src/Isosurface.cu:408-420) with the first pass of both scans
-    const GridDesc g = grid_desc(s, sh, m, gc_float3{1, 1, 1}, gc_float3{0, 0, 0});
-    hipLaunchKernelGGL(classify_tile_kernel, dim3(nb), dim3(kScanThreads), 0, st, d_verts, d_occ,
-                       vol, g, n, iso, t->num_verts, bsum_v, bsum_o);
-    GC_CHECK_LAUNCH();
-    // the occupancy scan also writes the compacted voxel list (nothing when
-    // no voxel is active, as the reference's early return leaves it)
-    if ((r = scan_finish(d_occ_scan, d_occ, n, bsum_o, nb, st, d_comp))) return r;
-    uint32_t last[2] = {0, 0};
-    if ((r = read_last_pair(d_occ, d_occ_scan, n, last, st, hw))) return r;
-    *active = last[0] + last[1];
-    if (*active == 0) {
-        *total = 0;
-        return GPUCAD_OK;
-    }
-    GC_ARG(d_verts_scan && d_comp, "NULL buffer");
-    if ((r = scan_finish(d_verts_scan, d_verts, n, bsum_v, nb, st, nullptr))) return r;
-    if ((r = read_last_pair(d_verts, d_verts_scan, n, last, st, hw))) return r;
-    *total = last[0] + last[1];
     GenArgs a;
     a.pos = (float4 *)pos;
     a.norm = (float4 *)norm;
@@ -592,11 +1069,20 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
     a.hi1 = hi1;
     a.lo2 = lo2;
     a.hi2 = hi2;
-    a.active = *active;
+    a.active = 0;
+    a.active_dev = w.tot;
+    a.vbase = w.vbase;
     a.max_verts = max_verts;
-    r = generate<MODE>(a, st, t);
-    if (r) return r;
+    a.tri = t->tri_table;
+    a.nverts = t->num_verts;
+    if (!later_null) {  // (otherwise no voxel is active)
+        hipLaunchKernelGGL(generate_kernel<MODE>, dim3(dropin_generate_blocks(n)), dim3(64), 0,
+                           st, a);
+        GC_CHECK_LAUNCH();
+    }
     GC_HIP(hipStreamSynchronize(st));
+    *active = hw[0];
+    *total = *active ? hw[1] : 0u;
     return GPUCAD_OK;
 }
 
@@ -631,6 +1117,36 @@ int noalloc_setup() {
 
 int check_dropin_grid(gc_uint3 s, gc_uint3 sh, gc_uint3 m, uint32_t n) {
     return check_grid(s, sh, m, n);
+}
+int dropin_scratch(gc_uint3 s, uint32_t n, DropinScratch *out) {
+    DropinScratch w = {};
+    const uint64_t np = (uint64_t)s.x * s.y * s.z;
+    w.nb = (uint32_t)(((uint64_t)n + kScanTile - 1) / kScanTile);
+    w.bits_blocks = (uint32_t)((np + kBitsBlockPoints - 1) / kBitsBlockPoints);
+    const size_t bits_u64 = (size_t)w.bits_blocks * (kBitsBlockPoints / 64) + 1;
+    void *ws = nullptr;
+    int r = scratch_arena(bits_u64 * 8 + (2 * (size_t)w.nb + 2 + (size_t)n) * 4, &ws);
+    if (r) return r;
+    w.bits = (uint64_t *)ws;
+    w.bsum_o = (uint32_t *)(w.bits + bits_u64);
+    w.bsum_v = w.bsum_o + w.nb;
+    w.tot = w.bsum_v + w.nb;
+    w.vbase = w.tot + 2;
+    *out = w;
+    return GPUCAD_OK;
+}
+int dropin_classify_scans(const DropinScratch &w, uint32_t *verts, uint32_t *occ,
+                          uint32_t *occ_scan, uint32_t *verts_scan, uint32_t *comp, gc_uint3 s,
+                          gc_uint3 sh, gc_uint3 m, uint32_t n, uint32_t *hw, hipStream_t st) {
+    const DeviceTables *t = nullptr;
+    int r = device_tables(&t);
+    if (r) return r;
+    return classify_scans(w, verts, occ, occ_scan, verts_scan, comp, s, sh, m, n, t, hw, false,
+                          st);
+}
+uint32_t dropin_generate_blocks(uint32_t n) {
+    const uint64_t max_batches = ((uint64_t)n + 63) / 64;
+    return (uint32_t)std::min<uint64_t>(kGenerateBlocks, (max_batches + 7) / 8 * 8);
 }
 int dropin_scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t st,
                     uint32_t *comp) {

@@ -66,6 +66,28 @@ int dropin_scan_u32(uint32_t *out, const uint32_t *in, uint32_t n, hipStream_t s
 int dropin_read_last_pair(const uint32_t *a, const uint32_t *b, uint32_t n, uint32_t *h,
                           hipStream_t st);
 
+// The orchestrations' classification and scans.  Bit p of `bits` (uint32
+// words, bit p % 32 of word p / 32) is point p's side for the cube index;
+// dropin_classify_scans classifies the n cells from it, scans occupancy and
+// vertex counts, compacts the active cells (with each one's vertex offset in
+// vbase) and leaves the totals (active voxels, vertices) in tot[0..1] and in
+// host words 0-1.  With no active cell, verts_scan, comp and vbase are not
+// written (and may be NULL).
+constexpr int kDropinBitsWaveWords = 16;  // 64-bit words of point bits per wave
+struct DropinScratch {
+    uint64_t *bits;          // the point bits (written by bits_blocks x 256 threads,
+                             // 64 x kDropinBitsWaveWords points per wave), then a padding word
+    uint32_t *bsum_o, *bsum_v, *tot, *vbase;
+    uint32_t bits_blocks, nb;
+};
+int dropin_scratch(gc_uint3 s, uint32_t n, DropinScratch *out);  // from the arena
+int dropin_classify_scans(const DropinScratch &w, uint32_t *verts, uint32_t *occ,
+                          uint32_t *occ_scan, uint32_t *verts_scan, uint32_t *comp, gc_uint3 s,
+                          gc_uint3 sh, gc_uint3 m, uint32_t n, uint32_t *hw, hipStream_t st);
+// workgroups of a persistent generate launch over at most n active cells
+// (a multiple of the 8 XCDs)
+uint32_t dropin_generate_blocks(uint32_t n);
+
 // Scratch memory owned by the library for drop-in calls (grown on demand,
 // one arena per device, never shrunk).  Callers hold lock_device_calls().
 int scratch_arena(size_t bytes, void **out);
