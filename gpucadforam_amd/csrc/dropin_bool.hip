@@ -278,31 +278,37 @@ __device__ __forceinline__ float merge_t(float t1, float t2, float t) {
     return t;
 }
 
+// The interpolations below compute each quotient unconditionally and
+// select it where the reference's branch would take it (the same values,
+// without divergent branches).
+
 // src/MarchingCubes_kernel.cu:1640-1671
-__device__ float3 interp_primitive(float iso, float3 p0, float3 p1, float f0, float f1, float te) {
-    float t2 = 0.0f;
-    if (straddles(f0, f1, iso)) t2 = (iso - f0) / (f1 - f0);
+__device__ __forceinline__ float3 interp_primitive(float iso, float3 p0, float3 p1, float f0,
+                                                   float f1, float te) {
+    const float q = (iso - f0) / (f1 - f0);
+    const float t2 = straddles(f0, f1, iso) ? q : 0.0f;
     return lerp3(p0, p1, merge_t(te, t2, 0.0f));
 }
 
 // src/MarchingCubes_kernel.cu:1674-1726
-__device__ float3 interp_primitive_one(float lo, float hi, float3 p0, float3 p1, float f0,
-                                       float f1, float te) {
-    float t2 = 0.0f, t3 = 0.0f;
-    if (straddles(f0, f1, lo)) t2 = (lo - f0) / (f1 - f0);
-    float t = merge_t(te, t2, 0.0f);
-    if (straddles(f0, f1, hi)) t3 = (hi - f0) / (f1 - f0);
-    t = merge_t(te, t3, t);
-    return lerp3(p0, p1, t);
+__device__ __forceinline__ float3 interp_primitive_one(float lo, float hi, float3 p0, float3 p1,
+                                                       float f0, float f1, float te) {
+    const float d = f1 - f0;
+    const float ql = (lo - f0) / d, qh = (hi - f0) / d;
+    const float t2 = straddles(f0, f1, lo) ? ql : 0.0f;
+    const float t3 = straddles(f0, f1, hi) ? qh : 0.0f;
+    return lerp3(p0, p1, merge_t(te, t3, merge_t(te, t2, 0.0f)));
 }
 
 // src/MarchingCubes_kernel.cu:1729-1784
-__device__ float3 interp_new(float iso, float lo, float hi, float3 p0, float3 p1, float f0, float f1,
-                             float f2, float f3) {
-    float t1 = 0.0f, t2 = 0.0f, t3 = 0.0f, t = 0.0f;
-    if (((f0 < iso) && (f1 >= iso)) || ((f1 < iso) && (f0 >= iso))) t1 = (iso - f0) / (f1 - f0);
-    if (((f2 < lo) && (f3 >= lo)) || ((f3 < lo) && (f2 >= lo))) t2 = (lo - f2) / (f3 - f2);
-    if (((f2 < hi) && (f3 >= hi)) || ((f3 < hi) && (f2 >= hi))) t3 = (hi - f2) / (f3 - f2);
+__device__ __forceinline__ float3 interp_new(float iso, float lo, float hi, float3 p0, float3 p1,
+                                             float f0, float f1, float f2, float f3) {
+    const float q1 = (iso - f0) / (f1 - f0);
+    const float q2 = (lo - f2) / (f3 - f2), q3 = (hi - f2) / (f3 - f2);
+    const float t1 = (((f0 < iso) && (f1 >= iso)) || ((f1 < iso) && (f0 >= iso))) ? q1 : 0.0f;
+    const float t2 = (((f2 < lo) && (f3 >= lo)) || ((f3 < lo) && (f2 >= lo))) ? q2 : 0.0f;
+    const float t3 = (((f2 < hi) && (f3 >= hi)) || ((f3 < hi) && (f2 >= hi))) ? q3 : 0.0f;
+    float t = 0.0f;
     if ((t1 > 0.0f) && (t2 > 0.0f) && (t3 == 0.0f)) t = (t1 + t2) * 0.5f;
     else if ((t1 > 0.0f) && (t3 > 0.0f) && (t2 == 0.0f)) t = (t1 + t3) * 0.5f;
     else if ((t1 > 0.0f) && (t2 == 0.0f) && (t3 == 0.0f)) t = t1;

@@ -613,63 +613,54 @@ __device__ __forceinline__ float3 cross3(float3 a, float3 b) {
 }
 
 // One band (src/MarchingCubes_kernel.cu:3593-3674 and the two halves of
-// 3269-3416): with f0 <= f1 after the swap, lo then hi level, endpoint snapping
-// at 5e-4.  Returns false when no branch applies (t stays as it was).
-__device__ __forceinline__ bool band_t(float lo, float hi, float3 &p0, float3 &p1, float f0,
-                                       float f1, float &t, float3 &snap, bool &snapped) {
-    if (f1 < f0) {
-        float3 tp = p1; p1 = p0; p0 = tp;
-        float tf = f1; f1 = f0; f0 = tf;
-    }
-    if ((f1 >= lo) && (f0 <= lo)) {
-        if (fabsf(lo - f0) < 0.0005f) { snap = p0; snapped = true; return true; }
-        if (fabsf(lo - f1) < 0.0005f) { snap = p1; snapped = true; return true; }
-        if (fabsf(f1 - f0) < 0.0005f) { snap = p0; snapped = true; return true; }
-        t = (lo - f0) / (f1 - f0);
-        return true;
-    }
-    if ((f1 >= hi) && (f0 <= hi)) {
-        if (fabsf(hi - f0) < 0.0005f) { snap = p0; snapped = true; return true; }
-        if (fabsf(hi - f1) < 0.0005f) { snap = p1; snapped = true; return true; }
-        if (fabsf(f1 - f0) < 0.0005f) { snap = p0; snapped = true; return true; }
-        t = (hi - f0) / (f1 - f0);
-        return true;
-    }
-    if (f1 == f0) {
-        t = (p0.z == 0.0f) ? 1.0f : 0.0f;
-        return true;
-    }
-    return false;
+// 3269-3416), on the edge from pa to pb with field values fa, fb, when `on`
+// (the indicator pair selects the band): the endpoints are ordered so that
+// f0 <= f1 (swapped when fb < fa); the lo level applies if f0 <= lo <= f1,
+// else the hi level if f0 <= hi <= f1; at the level, the vertex snaps to the
+// first endpoint when the level is within 5e-4 of f0, else to the second
+// when within 5e-4 of f1, else to the first when f1 - f0 < 5e-4, and lies at
+// t = (level - f0) / (f1 - f0) otherwise; with neither level, f1 == f0 gives
+// t = (first endpoint's z == 0 ? 1 : 0) and anything else t = 0.  Without
+// the band, t = 0 on the unordered endpoints.  Evaluated without branches
+// (one division per edge, every lane busy), the same values the reference's
+// branches give.
+__device__ __forceinline__ float3 band_vertex(float3 pa, float3 pb, float fa, float fb, float lo,
+                                              float hi, bool on) {
+    const bool sw = fb < fa;
+    const float f0 = sw ? fb : fa, f1 = sw ? fa : fb;
+    const float3 P0 = sw ? pb : pa, P1 = sw ? pa : pb;
+    const bool cl = (f1 >= lo) && (f0 <= lo);
+    const bool hit = cl || ((f1 >= hi) && (f0 <= hi));
+    const float lev = cl ? lo : hi;
+    const bool s0 = fabsf(lev - f0) < 0.0005f;
+    const bool s1 = fabsf(lev - f1) < 0.0005f;
+    const bool s2 = fabsf(f1 - f0) < 0.0005f;
+    const float td = (lev - f0) / (f1 - f0);
+    const float tz = (f1 == f0) ? (P0.z == 0.0f ? 1.0f : 0.0f) : 0.0f;
+    const float t = hit ? td : tz;
+    const float3 q0 = on ? P0 : pa, q1 = on ? P1 : pb;
+    const float3 r = lerp3(q0, q1, on ? t : 0.0f);
+    if (on && hit && !s0 && s1) return P1;
+    if (on && hit && (s0 || s2)) return P0;
+    return r;
 }
 
 // vertexInterp3_new (two fields; indicator values 1/0 use field one, 2/0 field two)
-__device__ float3 interp3(float lo1, float hi1, float3 p0, float3 p1, float f0, float f1, float g0,
-                          float g1, float lo2, float hi2, float id0, float id1) {
-    float t = 0.0f;
-    float3 snap;
-    bool snapped = false;
-    if ((id0 == 1.f && id1 == 0.f) || (id0 == 0.f && id1 == 1.f)) {
-        band_t(lo1, hi1, p0, p1, f0, f1, t, snap, snapped);
-        if (snapped) return snap;
-    }
-    if ((id0 == 2.f && id1 == 0.f) || (id1 == 2.f && id0 == 0.f)) {
-        band_t(lo2, hi2, p0, p1, g0, g1, t, snap, snapped);
-        if (snapped) return snap;
-    }
-    return lerp3(p0, p1, t);
+__device__ __forceinline__ float3 interp3(float lo1, float hi1, float3 p0, float3 p1, float f0,
+                                          float f1, float g0, float g1, float lo2, float hi2,
+                                          float id0, float id1) {
+    const bool on1 = (id0 == 1.f && id1 == 0.f) || (id0 == 0.f && id1 == 1.f);
+    const bool on2 = (id0 == 2.f && id1 == 0.f) || (id1 == 2.f && id0 == 0.f);
+    // (the two selections exclude each other)
+    return band_vertex(p0, p1, on2 ? g0 : f0, on2 ? g1 : f1, on2 ? lo2 : lo1, on2 ? hi2 : hi1,
+                       on1 || on2);
 }
 
 // vertexInterp2_new (one field)
-__device__ float3 interp2(float lo, float hi, float3 p0, float3 p1, float f0, float f1, float id0,
-                          float id1) {
-    float t = 0.0f;
-    float3 snap;
-    bool snapped = false;
-    if ((id0 == 1.f && id1 == 0.f) || (id0 == 0.f && id1 == 1.f)) {
-        band_t(lo, hi, p0, p1, f0, f1, t, snap, snapped);
-        if (snapped) return snap;
-    }
-    return lerp3(p0, p1, t);
+__device__ __forceinline__ float3 interp2(float lo, float hi, float3 p0, float3 p1, float f0,
+                                          float f1, float id0, float id1) {
+    const bool on = (id0 == 1.f && id1 == 0.f) || (id0 == 0.f && id1 == 1.f);
+    return band_vertex(p0, p1, f0, f1, lo, hi, on);
 }
 
 // vertexInterp4 (textbook)
