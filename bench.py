@@ -7,6 +7,10 @@ by-product) and extract its isosurface into indexed vertex/index buffers
 (marching cubes with shared-edge dedup).  Inputs are resident in HBM; output
 buffers are preallocated from a warm-up run's counts.
 
+Before the warm-up steps, untimed steps run in chunks of 10 until the time
+per step is steady (the GPU's clocks ramp up over the first tens of
+milliseconds of load); the timed steps then measure the steady state.
+
 Prints one JSON line:
   value       Gvoxels/s = grid points / (field + extraction) time per step
   roofline    the dominant kernel (field kernel, VALU-bound): the FP32 flops it
@@ -142,6 +146,35 @@ class HipEvent:
     def __del__(self):
         if self._hip is not None and self.e:
             self._hip.hipEventDestroy(self.e)
+
+
+def settle(pipe, stream, chunk=10, min_s=0.5, max_s=3.0, tol=0.01):
+    """Runs untimed steps until the GPU's clocks have ramped up: chunks of
+    `chunk` steps, each timed, for at least `min_s` seconds and then until the
+    last three chunks' times per step agree within `tol` (at most `max_s`
+    seconds).  A GPU coming out of idle runs its first tens of milliseconds
+    of work at lower clocks (the first 20 steps of the 512^3 step read 0.43 ms
+    per step, the steady state 0.36 ms).  Returns the number of chunks and the
+    first and last chunks' times per step (ms)."""
+    times = []
+    t0 = time.perf_counter()
+    while True:
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(chunk):
+            pipe.field_pass()
+            pipe.extract_pass()
+        b.record(stream)
+        b.synchronize()
+        times.append(a.elapsed_time(b) / chunk)
+        el = time.perf_counter() - t0
+        last = times[-3:]
+        if el >= max_s or (el >= min_s and len(last) == 3 and
+                           max(last) - min(last) <= tol * min(last)):
+            break
+    return {"chunks": len(times), "steps_per_chunk": chunk, "first_ms": round(times[0], 4),
+            "last_ms": round(times[-1], 4)}
 
 
 def timed_steps(pipe, stream, steps):
@@ -542,7 +575,8 @@ def cpu_baseline(p, slab, threads, budget_s=12.0):
         "cpu_model": cpu_model(),
         "nproc": os.cpu_count(),
         "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
-        "build": "g++ -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off (oracle/Makefile)",
+        "build": "g++ -O3 -march=%s -fopenmp -ffp-contract=off (oracle/Makefile; the ISA "
+                 "level of this host's CPU)" % orc.ISA,
         "kind": "port",
         "sample": "median of %d z-slabs of %d planes of the %dx%dx%d workload (%d points each, "
                   "%.1f s total): oracle field %.3f s + indexed extraction %.3f s per slab"
@@ -696,6 +730,7 @@ def main():
     p = CF.varying_schwarz_p(args.n)
     stream = torch.cuda.Stream()
     pipe = Pipeline(p, stream)
+    settled = settle(pipe, stream)
     for _ in range(args.warmup):
         pipe.field_pass()
         pipe.extract_pass()
@@ -760,6 +795,9 @@ def main():
         "n_vertices": counts[0],
         "n_triangles": counts[1],
         "wall_s": round(wall, 3),
+        # untimed chunks of 10 steps run before the warm-up steps until the
+        # time per step is steady (the GPU's clocks ramp up under load)
+        "settle": settled,
     }
     if not args.no_cpu:
         rec["fp64_error"] = fp64_errors(p, pipe.field, p.nz // 2 - 16, 33)

@@ -186,13 +186,6 @@ __device__ __forceinline__ Cross row_cross(uint64_t cur, uint64_t nxt, uint64_t 
     return c;
 }
 
-// a cell's corners all agree iff a == sa == b == sb == c == sc == d == sd at its bit
-__device__ __forceinline__ uint64_t active_cells(uint64_t a, uint64_t sa, uint64_t b, uint64_t sb,
-                                                 uint64_t c, uint64_t sc, uint64_t d, uint64_t sd,
-                                                 uint64_t ved) {
-    return ((a ^ sa) | (a ^ b) | (a ^ sb) | (a ^ c) | (a ^ sc) | (a ^ d) | (a ^ sd)) & ved;
-}
-
 // ---------------------------------------------------------------------------
 // grids and block scheduling
 // ---------------------------------------------------------------------------
@@ -251,23 +244,27 @@ signs_kernel(const float *__restrict__ field, uint64_t *__restrict__ signs, int 
 // body-diagonal cases, which have two (characteristic 0).  The identity is
 // checked case by case against the table by the tests.  So a word's count is
 // a weighted sum of popcounts over its 64 cells, with no per-cell lookup.
+// The sums run over all cells of the word, not only the active ones: a cell
+// with every corner outside adds 0 to each of them, and one with every corner
+// inside adds 8 - 2 * 6 = -4, which the last term takes back.  Corner bits of
+// cells past the end of the row must be 0 (the caller masks them).
 __device__ __forceinline__ uint32_t word_triangles(uint64_t k0, uint64_t k1, uint64_t k2,
                                                    uint64_t k3, uint64_t k4, uint64_t k5,
-                                                   uint64_t k6, uint64_t k7, uint64_t act) {
-    const uint32_t v = popc64(k0 & act) + popc64(k1 & act) + popc64(k2 & act) +
-                       popc64(k3 & act) + popc64(k4 & act) + popc64(k5 & act) +
-                       popc64(k6 & act) + popc64(k7 & act);
+                                                   uint64_t k6, uint64_t k7) {
+    const uint32_t v = popc64(k0) + popc64(k1) + popc64(k2) + popc64(k3) + popc64(k4) +
+                       popc64(k5) + popc64(k6) + popc64(k7);
     const uint64_t p01 = k0 & k1, p23 = k2 & k3, p45 = k4 & k5, p67 = k6 & k7;
     const uint64_t p12 = k1 & k2, p56 = k5 & k6, p03 = k0 & k3, p47 = k4 & k7;
-    const uint32_t f = popc64(p01 & p23 & act) + popc64(p45 & p67 & act) +
-                       popc64(p01 & p45 & act) + popc64(p23 & p67 & act) +
-                       popc64(p12 & p56 & act) + popc64(p03 & p47 & act);
+    const uint64_t bottom = p01 & p23, top = p45 & p67;
+    const uint32_t f = popc64(bottom) + popc64(top) + popc64(p01 & p45) + popc64(p23 & p67) +
+                       popc64(p12 & p56) + popc64(p03 & p47);
     // body diagonals (0,6) (1,7) (2,4) (3,5): both ends outside, the rest inside
-    const uint32_t d = popc64(~(k0 | k6) & p12 & p45 & k3 & k7 & act) +
-                       popc64(~(k1 | k7) & p03 & p56 & k2 & k4 & act) +
-                       popc64(~(k2 | k4) & p01 & p67 & k3 & k5 & act) +
-                       popc64(~(k3 | k5) & p12 & p67 & k0 & k4 & act);
-    return v - 2 * f - 4 * d;
+    const uint32_t d = popc64(~(k0 | k6) & p12 & p45 & k3 & k7) +
+                       popc64(~(k1 | k7) & p03 & p56 & k2 & k4) +
+                       popc64(~(k2 | k4) & p01 & p67 & k3 & k5) +
+                       popc64(~(k3 | k5) & p12 & p67 & k0 & k4);
+    const uint32_t inside = popc64(bottom & top);
+    return v - 2 * f - 4 * d + 4 * inside;
 }
 
 struct CountArgs {
@@ -316,8 +313,8 @@ __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a) {
                   popc64(ez & 0xffffffffull);
             if (has_y && has_z) {
                 const uint64_t sd = shift1(wd, dn);
-                const uint64_t act = active_cells(wa, sa, wb, sb, wc, sc, wd, sd, ved);
-                tc = word_triangles(wa, sa, sb, wb, wc, sc, sd, wd, act);
+                tc = word_triangles(wa & ved, sa & ved, sb & ved, wb & ved, wc & ved, sc & ved,
+                                    sd & ved, wd & ved);
             }
         }
         const uint32_t vincl = wave_scan_incl(vc);
@@ -421,7 +418,8 @@ scan_blocks_kernel(uint64_t *bsum, uint64_t nb, uint64_t *ctl, uint64_t *d_count
     for (uint32_t i = t; i < nseg; i += 1024) {
         const uint64_t b0 = segs ? segs[i].block_off : 0;
         const uint64_t b1 = (segs && i + 1 < nseg) ? segs[i + 1].block_off : nb;
-        const uint64_t v0 = bsum[2 * b0], t0 = bsum[2 * b0 + 1];
+        // one grid: its first block's prefix is 0 (no read back)
+        const uint64_t v0 = segs ? bsum[2 * b0] : 0ull, t0 = segs ? bsum[2 * b0 + 1] : 0ull;
         const uint64_t v1 = b1 < nb ? bsum[2 * b1] : carry_v;
         const uint64_t t1 = b1 < nb ? bsum[2 * b1 + 1] : carry_t;
         SegResult r;
@@ -454,20 +452,18 @@ scan_blocks_kernel(uint64_t *bsum, uint64_t nb, uint64_t *ctl, uint64_t *d_count
 __device__ __forceinline__ V3 edge_vertex(const Seg &s, float xf, float yf, float zf, uint32_t ax,
                                           float f0, float f1) {
     const float t = (s.iso - f0) / (f1 - f0);
-    float px = __builtin_fmaf(xf, s.sx, s.ox);
-    float py = __builtin_fmaf(yf, s.sy, s.oy);
-    float pz = __builtin_fmaf(zf, s.sz, s.oz);
-    if (ax == 0) {
-        const float p1 = __builtin_fmaf(xf + 1.0f, s.sx, s.ox);
-        px = __builtin_fmaf(t, p1 - px, px);
-    } else if (ax == 1) {
-        const float p1 = __builtin_fmaf(yf + 1.0f, s.sy, s.oy);
-        py = __builtin_fmaf(t, p1 - py, py);
-    } else {
-        const float p1 = __builtin_fmaf(zf + 1.0f, s.sz, s.oz);
-        pz = __builtin_fmaf(t, p1 - pz, pz);
-    }
-    return V3{px, py, pz};
+    const float px = __builtin_fmaf(xf, s.sx, s.ox);
+    const float py = __builtin_fmaf(yf, s.sy, s.oy);
+    const float pz = __builtin_fmaf(zf, s.sz, s.oz);
+    // the edge's axis by selects (lanes of a wave hold edges of all three
+    // axes): its low end p0, its high end p1, and p0 + t (p1 - p0) on it
+    const float c = ax == 0 ? xf : (ax == 1 ? yf : zf);
+    const float sc = ax == 0 ? s.sx : (ax == 1 ? s.sy : s.sz);
+    const float oc = ax == 0 ? s.ox : (ax == 1 ? s.oy : s.oz);
+    const float p0 = ax == 0 ? px : (ax == 1 ? py : pz);
+    const float p1 = __builtin_fmaf(c + 1.0f, sc, oc);
+    const float pe = __builtin_fmaf(t, p1 - p0, p0);
+    return V3{ax == 0 ? pe : px, ax == 1 ? pe : py, ax == 2 ? pe : pz};
 }
 
 struct VertArgs {
@@ -710,9 +706,11 @@ struct TriShared {
 template <class Src>
 __device__ __forceinline__ void tri_block(const Src &src, const TriArgs &a, uint32_t tb,
                                           TriShared &sh) {
+    static_assert(kTriThreads == 256, "one case-table entry per thread");
     uint64_t *s_tri = sh.tri;
-    if (threadIdx.x < 256) s_tri[threadIdx.x] = a.tri_packed[threadIdx.x];
-    __syncthreads();
+    // the case table's entry is loaded first and stored to LDS after the
+    // sign loads below have returned: both share one round trip
+    const uint64_t tri_entry = a.tri_packed[threadIdx.x];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = tid & 1;
     {
         const uint32_t gb = tb >> 1, part = tb & 1;  // count block and its half
@@ -795,6 +793,8 @@ __device__ __forceinline__ void tri_block(const Src &src, const TriArgs &a, uint
                 }
             }
         }
+        s_tri[tid] = tri_entry;
+        __syncthreads();
         // vertex numbers that do not fit 32 bits: the grid's triangles are
         // not written
         const uint64_t tcap = ovf ? 0ull : a.tcap;

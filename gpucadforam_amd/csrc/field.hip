@@ -83,6 +83,8 @@ int validate_lattice(const gc_lattice_params *p) {
     GC_ARG(p, "params is NULL");
     GC_ARG(p->nx >= 2 && p->ny >= 2 && p->nz >= 2, "grid must have >= 2 points per axis");
     GC_ARG((uint64_t)p->nx * p->ny * p->nz < (1ull << 40), "grid too large");
+    // the field kernels address a z plane with 32-bit byte offsets
+    GC_ARG((uint64_t)p->nx * p->ny < (1ull << 30), "a z plane must hold fewer than 2^30 points");
     GC_ARG(p->tpms >= 0 && p->tpms <= 2, "unknown tpms %d", p->tpms);
     GC_ARG(p->period0 > 0.f && p->period1 > 0.f, "periods must be positive");
     GC_ARG(p->bend_radius >= 0.f, "bend radius must be >= 0");

@@ -89,17 +89,22 @@ __device__ __forceinline__ float atan2_poly(float y, float x) {
 // correctly rounded neighbour from the fma residuals of s - ulp and s + ulp:
 //   s - ulp  if x - (s - ulp) s <= 0,   s + ulp  if x - (s + ulp) s > 0.
 // This equals sqrtf on that range (and on +inf, where both residuals are
-// NaN).  A wave with a lane below 2^-96 (zero and NaN included) takes
-// sqrtf itself, which rescales small inputs and patches zero.
+// NaN).  An input below 2^-96 (subnormals, zeros and negatives included) is
+// scaled by 2^64 first and its root by 2^-32: both products are exact, and
+// correct rounding commutes with them, so the result is still sqrtf(x)
+// (zeros keep their sign, negatives give NaN).  No branch: the callers
+// evaluate several independent points side by side.
 __device__ __forceinline__ float sqrt_rn(float x) {
-    if (__ballot(!(x >= 0x1p-96f)) != 0) return __builtin_sqrtf(x);
-    const float s = __builtin_amdgcn_sqrtf(x);
+    const bool tiny = x < 0x1p-96f;
+    const float xs = tiny ? x * 0x1p64f : x;
+    const float s = __builtin_amdgcn_sqrtf(xs);
     const float sd = __uint_as_float(__float_as_uint(s) - 1u);
     const float su = __uint_as_float(__float_as_uint(s) + 1u);
-    const float rd = __builtin_fmaf(-sd, s, x);
-    const float ru = __builtin_fmaf(-su, s, x);
-    const float r = rd <= 0.0f ? sd : s;
-    return ru > 0.0f ? su : r;
+    const float rd = __builtin_fmaf(-sd, s, xs);
+    const float ru = __builtin_fmaf(-su, s, xs);
+    const float r0 = rd <= 0.0f ? sd : s;
+    const float r = ru > 0.0f ? su : r0;
+    return tiny ? r * 0x1p-32f : r;
 }
 
 }  // namespace gc

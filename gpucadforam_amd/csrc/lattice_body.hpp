@@ -157,6 +157,15 @@ __device__ __forceinline__ void store_sign_bits(uint64_t *row, int x0, uint64_t 
     }
 }
 
+// Buffer stores of the brick: word 3 of a gfx9 buffer resource for untyped
+// access, the offset given to lanes that must not store (past any plane:
+// planes are below 4 GiB, validated by the launches), and the non-temporal
+// cache-policy bit of the store.
+constexpr int kBufferWord3 = 0x00020000;
+constexpr uint32_t kPastPlane = 0xfffffff0u;
+constexpr int kStoreNonTemporal = 2;
+typedef uint32_t U32x2 __attribute__((ext_vector_type(2)));
+
 // Primitive of a lattice field known at compile time (PRIM_ANY: read from
 // the constants at run time).
 constexpr int PRIM_ANY = -1;
@@ -373,6 +382,9 @@ __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__res
         }
         const int prim = PRIM == PRIM_ANY ? c.prim : PRIM;
         const float pz = prim != GC_PRIM_NONE ? e[ZT_PRIMZ] : 0.f;
+        // every row's value first (straight-line code, so the rows' independent
+        // chains interleave), then the stores, which are guarded per row
+        float gv[RY];
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
             float f;
@@ -424,6 +436,41 @@ __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__res
                         ? gi
                         : (op == GC_OP_UNION ? gu : (op == GC_OP_SUBTRACT ? gs : g));
             }
+            gv[r] = g;
+        }
+        if (BX == 64) {
+            // buffer stores over this z plane: a lane whose point lies outside
+            // the grid (and every lane but the first for the sign word) gets
+            // an offset past the plane, which the buffer's range check
+            // discards, so the stores need no branch and the rows' arithmetic
+            // above stays one block
+            const __amdgpu_buffer_rsrc_t fr =
+                __builtin_amdgcn_make_buffer_rsrc(fplane + (uint64_t)i * fstep, (short)0,
+                                                  (int)(uint32_t)fstep, kBufferWord3);
+#pragma unroll
+            for (int r = 0; r < RY; ++r)
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(gv[r]), fr,
+                                                      (int)((xin && yin[r]) ? foff[r] : kPastPlane),
+                                                      0, kStoreNonTemporal);
+            if (splane) {
+                const __amdgpu_buffer_rsrc_t sr =
+                    __builtin_amdgcn_make_buffer_rsrc(splane + (uint64_t)i * sstep, (short)0,
+                                                      (int)(uint32_t)sstep, kBufferWord3);
+#pragma unroll
+                for (int r = 0; r < RY; ++r) {
+                    const uint64_t m = __builtin_amdgcn_fcmpf(gv[r], c.iso, 4) & in_mask;
+                    const U32x2 w = {(uint32_t)m, (uint32_t)(m >> 32)};
+                    __builtin_amdgcn_raw_buffer_store_b64(
+                        w, sr, (int)((sign_lane && yin[r]) ? soff[r] + 8u * (uint32_t)(x0 >> 6)
+                                                           : kPastPlane),
+                        0, 0);
+                }
+            }
+            continue;
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const float g = gv[r];
             // the field is written once per launch (the extraction reads it
             // back from HBM): non-temporal stores
             if (xin && yin[r])

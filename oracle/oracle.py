@@ -12,7 +12,29 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+V4_FLAGS = ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl")
+
+
+def host_isa():
+    """"x86-64-v4" when the host CPU has the AVX-512 extensions of that level
+    (from /proc/cpuinfo), else "x86-64-v3"."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    return "x86-64-v4" if all(x in flags for x in V4_FLAGS) else "x86-64-v3"
+    except OSError:
+        pass
+    return "x86-64-v3"
+
+
+# both builds compute the same bits (oracle/Makefile); the one for the host's
+# ISA level is loaded when it is built
+ISA = host_isa()
+LIB_PATH = os.path.join(HERE, "liboracle_v4.so" if ISA == "x86-64-v4" else "liboracle.so")
+if not os.path.exists(LIB_PATH):
+    ISA, LIB_PATH = "x86-64-v3", os.path.join(HERE, "liboracle.so")
 
 
 def build():

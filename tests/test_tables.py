@@ -139,3 +139,23 @@ def test_triangle_count_identity(tables):
         without_diagonal += nv[c] // 3 != v - 2 * f
     # the body-diagonal term matters in exactly its four cases
     assert without_diagonal == 4
+
+
+def test_triangle_count_over_all_cells(tables):
+    """The count stage sums |S| - 2F - 4D over every cell of a word, active or
+    not, and adds 4 per cell with all eight corners inside: the all-outside
+    case contributes 0 and the all-inside case exactly -4, so the word total
+    equals the table's."""
+    tri, nv, em = tables
+    assert count_terms(0) == (0, 0, 0)
+    v, f, d = count_terms(255)
+    assert (v, f, d) == (8, 6, 0) and v - 2 * f - 4 * d == -4 and nv[255] == 0
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        cases = rng.integers(0, 256, size=64)
+        cases[rng.integers(0, 64, size=8)] = 255  # runs of inside cells
+        total = 0
+        for c in cases:
+            v, f, d = count_terms(int(c))
+            total += v - 2 * f - 4 * d + (4 if c == 255 else 0)
+        assert total == sum(int(nv[c]) // 3 for c in cases)
