@@ -698,8 +698,8 @@ struct TriShared {
     uint64_t tri[256];
     uint16_t list[kWaves][kCellCap];
     uint32_t vid[kWaves][12 * 64];  // id k of lane l at k*64 + (l ^ k)
-    uint64_t pk[kWaves][64];        // packed triangle list of each lane's cell
-    uint16_t own[kWaves][5 * 64];   // triangle -> lane | (triangle of the cell) << 8
+    // triangle -> lane | its three edges (4 bits each) << 8
+    uint32_t own[kWaves][5 * 64];
 };
 
 // Triangles of triangle block tb (half tb & 1 of count block tb >> 1).
@@ -811,8 +811,7 @@ __device__ __forceinline__ void tri_block(const Src &src, const TriArgs &a, uint
         const uint32_t aex = aincl - nact;
         uint16_t *list = sh.list[wave];
         uint32_t *vid = sh.vid[wave];
-        uint64_t *pk = sh.pk[wave];
-        uint16_t *own = sh.own[wave];
+        uint32_t *own = sh.own[wave];
         for (uint32_t r0 = 0; r0 < atot; r0 += kCellCap) {
             fill_cells(list, act, nact, aex, atot, r0, lane);
             const uint32_t n = min((uint32_t)kCellCap, atot - r0);
@@ -886,20 +885,20 @@ __device__ __forceinline__ void tri_block(const Src &src, const TriArgs &a, uint
                 const uint32_t tincl = wave_scan_incl(nt);
                 const uint32_t tex = tincl - nt, ttot = readlane32(tincl, 63);
                 if (on) {
-                    pk[lane] = packed;
-                    for (uint32_t jt = 0; jt < nt; ++jt) own[tex + jt] = (uint16_t)(lane | (jt << 8));
+                    uint64_t rest = packed;
+                    for (uint32_t jt = 0; jt < nt; ++jt, rest >>= 12)
+                        own[tex + jt] = lane | (((uint32_t)rest & 0xfffu) << 8);
                 }
                 __builtin_amdgcn_wave_barrier();
                 // the wave's triangles, one per lane: consecutive lanes store
                 // consecutive 12-byte records
                 for (uint32_t te = lane; te < ttot; te += 64) {
-                    const uint32_t o = own[te], l = o & 255, jt = o >> 8;
-                    const uint64_t pkl = pk[l];
+                    const uint32_t o = own[te], l = o & 63u;
                     const uint64_t t = tnext + te;
                     if (t < tcap) {
-                        const uint32_t e0v = (uint32_t)(pkl >> (12 * jt)) & 15u;
-                        const uint32_t e1v = (uint32_t)(pkl >> (12 * jt + 4)) & 15u;
-                        const uint32_t e2v = (uint32_t)(pkl >> (12 * jt + 8)) & 15u;
+                        const uint32_t e0v = (o >> 8) & 15u;
+                        const uint32_t e1v = (o >> 12) & 15u;
+                        const uint32_t e2v = (o >> 16) & 15u;
                         const uint32_t v0 = vid[e0v * 64 + (l ^ e0v)];
                         const uint32_t v1 = vid[e1v * 64 + (l ^ e1v)];
                         const uint32_t v2 = vid[e2v * 64 + (l ^ e2v)];
