@@ -87,6 +87,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--settle", type=float, default=0.5,
+                    help="seconds of untimed steps before the warm-up (0: none, for profiler runs)")
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -177,10 +179,11 @@ def settle(pipe, stream, chunk=10, min_s=0.5, max_s=3.0, tol=0.01):
             "last_ms": round(times[-1], 4)}
 
 
-def timed_steps(pipe, stream, steps):
+def timed_steps(pipe, stream, steps, rounds=3):
     """(ms per step, field ms, extraction ms): first `steps` steps bracketed
-    by two events only, then as many again with a fenceless event before and
-    after each pass for the per-pass times."""
+    by two events only, then `rounds` times as many again with a fenceless
+    event before and after each pass; each pass's time is the median over the
+    rounds of its mean per step (one round can be disturbed on its own)."""
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -191,18 +194,20 @@ def timed_steps(pipe, stream, steps):
     end.record(stream)
     torch.cuda.synchronize()
     ms = start.elapsed_time(end) / steps
-    ev = [[HipEvent() for _ in range(3)] for _ in range(steps)]
     h = pipe.h
-    for k in range(steps):
-        ev[k][0].record(h)
-        pipe.field_pass()
-        ev[k][1].record(h)
-        pipe.extract_pass()
-        ev[k][2].record(h)
-    torch.cuda.synchronize()
-    field_ms = sum(e[0].ms_to(e[1]) for e in ev) / steps
-    extract_ms = sum(e[1].ms_to(e[2]) for e in ev) / steps
-    return ms, field_ms, extract_ms
+    field, extract = [], []
+    for _ in range(rounds):
+        ev = [[HipEvent() for _ in range(3)] for _ in range(steps)]
+        for k in range(steps):
+            ev[k][0].record(h)
+            pipe.field_pass()
+            ev[k][1].record(h)
+            pipe.extract_pass()
+            ev[k][2].record(h)
+        torch.cuda.synchronize()
+        field.append(sum(e[0].ms_to(e[1]) for e in ev) / steps)
+        extract.append(sum(e[1].ms_to(e[2]) for e in ev) / steps)
+    return ms, statistics.median(field), statistics.median(extract)
 
 
 class Pipeline:
@@ -734,7 +739,7 @@ def main():
     p = CF.varying_schwarz_p(args.n)
     stream = torch.cuda.Stream()
     pipe = Pipeline(p, stream)
-    settled = settle(pipe, stream)
+    settled = settle(pipe, stream, min_s=args.settle) if args.settle > 0 else None
     for _ in range(args.warmup):
         pipe.field_pass()
         pipe.extract_pass()

@@ -351,47 +351,54 @@ __global__ void __launch_bounds__(kBlock) count_kernel(Src src, CountArgs a) {
 __global__ void __launch_bounds__(1024)
 scan_blocks_kernel(uint64_t *bsum, uint64_t nb, uint64_t *ctl, uint64_t *d_counts,
                    const Seg *segs, uint32_t nseg, SegResult *res) {
-    // thread t owns the contiguous entries [t*per, (t+1)*per): it sums them,
-    // the workgroup scans the 1024 sums, then the thread writes its entries'
-    // exclusive prefixes (one pass over the block sums, one barrier round)
-    __shared__ uint64_t sw_v[16], sw_t[16];
+    // Wave w scans the contiguous entries [w*per, (w+1)*per) in steps of 64
+    // (lane l takes entry 64*step + l: coalesced 16-byte loads and stores).
+    // A block's sums (at most 256 words x 192 crossing edges, or 256 x 64
+    // cells x 5 triangles) and a step's (64 of them) fit 32 bits, so each
+    // step is a DPP wave scan of 32-bit values with 64-bit carries.  The
+    // first kKeep steps stay in registers for the write-back (8192 blocks:
+    // a 512^3 grid); steps past them are read again and scanned again.
+    constexpr int kWaves = 16, kKeep = 8;
+    static_assert((uint64_t)64 * kBlock * 64 * 5 < (1ull << 32), "a step's sums fit 32 bits");
+    __shared__ uint64_t sw_v[kWaves], sw_t[kWaves];
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     ulonglong2 *io = (ulonglong2 *)bsum;
-    const uint64_t per = (nb + 1023) / 1024;
-    const uint64_t b0 = min(nb, (uint64_t)t * per), b1 = min(nb, b0 + per);
-    // entries are read kScanRun at a time, all loads of a run in flight; the
-    // first run stays in registers for the write-back
-    constexpr int kScanRun = 8;
-    ulonglong2 keep[kScanRun];
-    uint64_t sv = 0, st = 0;
-    for (uint64_t i0 = b0; i0 < b1; i0 += kScanRun) {
-        ulonglong2 q[kScanRun];
+    const uint64_t per = (nb + kWaves * 64 - 1) / (kWaves * 64) * 64;
+    const uint64_t w0 = min(nb, (uint64_t)wave * per), w1 = min(nb, w0 + per);
+    const uint32_t nsteps = (uint32_t)((w1 - w0 + 63) / 64);
+    uint32_t qv[kKeep], qt[kKeep];
 #pragma unroll
-        for (int u = 0; u < kScanRun; ++u)
-            q[u] = i0 + u < b1 ? io[i0 + u] : make_ulonglong2(0ull, 0ull);
-#pragma unroll
-        for (int u = 0; u < kScanRun; ++u) {
-            sv += q[u].x;
-            st += q[u].y;
-            if (i0 == b0) keep[u] = q[u];
-        }
+    for (int u = 0; u < kKeep; ++u) {
+        const uint64_t i = w0 + 64 * (uint64_t)u + lane;
+        const ulonglong2 q = i < w1 ? io[i] : make_ulonglong2(0ull, 0ull);
+        qv[u] = (uint32_t)q.x;
+        qt[u] = (uint32_t)q.y;
     }
-    uint64_t iv = sv, it = st;  // inclusive scan within the wave
+    uint32_t ev[kKeep], et[kKeep], tv[kKeep], tt[kKeep];  // exclusive scans, step totals
+    uint64_t sv = 0, st = 0;                              // the wave's totals
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint64_t pv = __shfl_up(iv, off, 64), pt = __shfl_up(it, off, 64);
-        if (lane >= (uint32_t)off) {
-            iv += pv;
-            it += pt;
-        }
+    for (int u = 0; u < kKeep; ++u) {
+        const uint32_t iv = wave_scan_incl(qv[u]), it = wave_scan_incl(qt[u]);
+        ev[u] = iv - qv[u];
+        et[u] = it - qt[u];
+        tv[u] = readlane32(iv, 63);
+        tt[u] = readlane32(it, 63);
+        sv += tv[u];
+        st += tt[u];
     }
-    if (lane == 63) {
-        sw_v[wave] = iv;
-        sw_t[wave] = it;
+    for (uint32_t u = kKeep; u < nsteps; ++u) {
+        const uint64_t i = w0 + 64 * (uint64_t)u + lane;
+        const ulonglong2 q = i < w1 ? io[i] : make_ulonglong2(0ull, 0ull);
+        sv += readlane32(wave_scan_incl((uint32_t)q.x), 63);
+        st += readlane32(wave_scan_incl((uint32_t)q.y), 63);
+    }
+    if (lane == 0) {
+        sw_v[wave] = sv;
+        sw_t[wave] = st;
     }
     __syncthreads();
-    uint64_t rv = iv - sv, rt = it - st, carry_v = 0, carry_t = 0;
-    for (uint32_t w = 0; w < 16; ++w) {
+    uint64_t rv = 0, rt = 0, carry_v = 0, carry_t = 0;
+    for (uint32_t w = 0; w < kWaves; ++w) {
         if (w < wave) {
             rv += sw_v[w];
             rt += sw_t[w];
@@ -399,18 +406,20 @@ scan_blocks_kernel(uint64_t *bsum, uint64_t nb, uint64_t *ctl, uint64_t *d_count
         carry_v += sw_v[w];
         carry_t += sw_t[w];
     }
-    for (uint64_t i0 = b0; i0 < b1; i0 += kScanRun) {
-        ulonglong2 q[kScanRun];
 #pragma unroll
-        for (int u = 0; u < kScanRun; ++u)
-            q[u] = i0 == b0 ? keep[u]
-                            : (i0 + u < b1 ? io[i0 + u] : make_ulonglong2(0ull, 0ull));
-#pragma unroll
-        for (int u = 0; u < kScanRun; ++u) {
-            if (i0 + u < b1) io[i0 + u] = make_ulonglong2(rv, rt);
-            rv += q[u].x;
-            rt += q[u].y;
-        }
+    for (int u = 0; u < kKeep; ++u) {
+        const uint64_t i = w0 + 64 * (uint64_t)u + lane;
+        if (i < w1) io[i] = make_ulonglong2(rv + ev[u], rt + et[u]);
+        rv += tv[u];
+        rt += tt[u];
+    }
+    for (uint32_t u = kKeep; u < nsteps; ++u) {
+        const uint64_t i = w0 + 64 * (uint64_t)u + lane;
+        const ulonglong2 q = i < w1 ? io[i] : make_ulonglong2(0ull, 0ull);
+        const uint32_t iv = wave_scan_incl((uint32_t)q.x), it = wave_scan_incl((uint32_t)q.y);
+        if (i < w1) io[i] = make_ulonglong2(rv + (iv - (uint32_t)q.x), rt + (it - (uint32_t)q.y));
+        rv += readlane32(iv, 63);
+        rt += readlane32(it, 63);
     }
     __syncthreads();
     // every thread holds the totals; the scanned sums are visible to the workgroup
@@ -981,8 +990,7 @@ int enqueue_extract(const gc_extract_args *a, hipStream_t stream) {
     uint64_t *bsum = (uint64_t *)(ws + L.bsum_off);
     uint64_t *ctl = (uint64_t *)(ws + L.ctl_off);
     SegResult *res = (SegResult *)(ws + L.res_off);
-    const dim3 blocks((unsigned)L.nb);
-    hipLaunchKernelGGL(count_kernel<OneGrid>, blocks, dim3(kBlock), 0, stream, one,
+    hipLaunchKernelGGL(count_kernel<OneGrid>, dim3((unsigned)L.nb), dim3(kBlock), 0, stream, one,
                        CountArgs{signs, twave, bsum, base});
     GC_CHECK_LAUNCH();
     hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, stream, bsum, L.nb, ctl,
