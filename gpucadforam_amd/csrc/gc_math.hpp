@@ -84,26 +84,31 @@ __device__ __forceinline__ float atan2_poly(float y, float x) {
     return t;
 }
 
-// Correctly rounded square root.  v_sqrt_f32 is within 1 ulp for inputs of
-// at least 2^-96; Tuckerman's rounding test (Markstein 1990) then picks the
-// correctly rounded neighbour from the fma residuals of s - ulp and s + ulp:
+// Correctly rounded square root of x >= 2^-96 (and of +inf and NaN).
+// v_sqrt_f32 is within 1 ulp on that range; Tuckerman's rounding test
+// (Markstein 1990) then picks the correctly rounded neighbour from the fma
+// residuals of s - ulp and s + ulp:
 //   s - ulp  if x - (s - ulp) s <= 0,   s + ulp  if x - (s + ulp) s > 0.
 // This equals sqrtf on that range (and on +inf, where both residuals are
-// NaN).  An input below 2^-96 (subnormals, zeros and negatives included) is
-// scaled by 2^64 first and its root by 2^-32: both products are exact, and
-// correct rounding commutes with them, so the result is still sqrtf(x)
-// (zeros keep their sign, negatives give NaN).  No branch: the callers
-// evaluate several independent points side by side.
-__device__ __forceinline__ float sqrt_rn(float x) {
-    const bool tiny = x < 0x1p-96f;
-    const float xs = tiny ? x * 0x1p64f : x;
-    const float s = __builtin_amdgcn_sqrtf(xs);
+// NaN), and at +0 and -0 (v_sqrt_f32 returns them; the residual of s - ulp
+// is NaN and that of s + ulp is not positive).
+__device__ __forceinline__ float sqrt_rn_normal(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
     const float sd = __uint_as_float(__float_as_uint(s) - 1u);
     const float su = __uint_as_float(__float_as_uint(s) + 1u);
-    const float rd = __builtin_fmaf(-sd, s, xs);
-    const float ru = __builtin_fmaf(-su, s, xs);
+    const float rd = __builtin_fmaf(-sd, s, x);
+    const float ru = __builtin_fmaf(-su, s, x);
     const float r0 = rd <= 0.0f ? sd : s;
-    const float r = ru > 0.0f ? su : r0;
+    return ru > 0.0f ? su : r0;
+}
+
+// Correctly rounded square root of any x: an input below 2^-96 (subnormals,
+// zeros and negatives included) is scaled by 2^64 first and its root by
+// 2^-32; both products are exact and correct rounding commutes with them, so
+// the result is still sqrtf(x) (zeros keep their sign, negatives give NaN).
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const bool tiny = x < 0x1p-96f;
+    const float r = sqrt_rn_normal(tiny ? x * 0x1p64f : x);
     return tiny ? r * 0x1p-32f : r;
 }
 

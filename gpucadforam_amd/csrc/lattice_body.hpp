@@ -170,6 +170,21 @@ typedef uint32_t U32x2 __attribute__((ext_vector_type(2)));
 // the constants at run time).
 constexpr int PRIM_ANY = -1;
 
+// The lattice value g combined with the primitive whose distance root is
+// `root` (sphere: root - r; box: root + min(max(partial, z term), 0)).
+template <int PRIM, int OP>
+__device__ __forceinline__ float prim_op(const FieldConsts &c, float g, float root, float pz,
+                                         float pbxy) {
+    const int prim = PRIM == PRIM_ANY ? c.prim : PRIM;
+    const float s = prim == GC_PRIM_SPHERE
+                        ? root - c.pr
+                        : root + __builtin_fminf(__builtin_fmaxf(pbxy, pz), 0.f);
+    const int op = OP == PRIM_ANY ? c.op : OP;
+    const float gi = __builtin_fmaxf(g, s), gu = __builtin_fminf(g, s), gs = __builtin_fmaxf(g, -s);
+    return op == GC_OP_INTERSECT ? gi
+                                 : (op == GC_OP_UNION ? gu : (op == GC_OP_SUBTRACT ? gs : g));
+}
+
 // Evaluates the brick with first point (x0, y0, z0) of the grid `c`.  Must be
 // called by all 256 threads of the workgroup (it synchronises through LDS);
 // `ztab` is kMaxLZ * kZTab floats of LDS.  `field` and `signs` point at the
@@ -385,6 +400,8 @@ __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__res
         // every row's value first (straight-line code, so the rows' independent
         // chains interleave), then the stores, which are guarded per row
         float gv[RY];
+        float gl[RY], px2[RY];  // the lattice value and the primitive's squared distance
+        bool small = false;     // a squared distance in (0, 2^-96) (or NaN)
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
             float f;
@@ -420,23 +437,25 @@ __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__res
                 t = __builtin_fmaf(-S3[r].im, E[5], t);
             }
             float g = c.sheet ? __builtin_fabsf(f) - t : f;
+            gl[r] = g;
+            px2[r] = 0.f;
             if (prim != GC_PRIM_NONE) {
-                float s;
-                if (prim == GC_PRIM_SPHERE) {
-                    s = sqrt_rn(__builtin_fmaf(pz, pz, pxy[r])) - c.pr;
-                } else {
-                    float mz = __builtin_fmaxf(pz, 0.f);
-                    s = sqrt_rn(__builtin_fmaf(mz, mz, pxy[r])) +
-                        __builtin_fminf(__builtin_fmaxf(pbxy[r], pz), 0.f);
-                }
-                const int op = OP == PRIM_ANY ? c.op : OP;
-                const float gi = __builtin_fmaxf(g, s), gu = __builtin_fminf(g, s),
-                            gs = __builtin_fmaxf(g, -s);
-                g = op == GC_OP_INTERSECT
-                        ? gi
-                        : (op == GC_OP_UNION ? gu : (op == GC_OP_SUBTRACT ? gs : g));
+                const float mz = __builtin_fmaxf(pz, 0.f);
+                px2[r] = prim == GC_PRIM_SPHERE ? __builtin_fmaf(pz, pz, pxy[r])
+                                                : __builtin_fmaf(mz, mz, pxy[r]);
+                // exact zeros (a box's inside) are exact on the fast path too
+                small |= !(px2[r] >= 0x1p-96f) && px2[r] != 0.0f;
+                g = prim_op<PRIM, OP>(c, g, sqrt_rn_normal(px2[r]), pz, pbxy[r]);
             }
             gv[r] = g;
+        }
+        // a squared distance in (0, 2^-96), a point within 2^-48 of the
+        // primitive's centre: the wave takes the rescaled root for every row
+        // (one uniform test per z step, after the rows' arithmetic)
+        if (prim != GC_PRIM_NONE && __ballot(small) != 0) {
+#pragma unroll
+            for (int r = 0; r < RY; ++r)
+                gv[r] = prim_op<PRIM, OP>(c, gl[r], sqrt_rn(px2[r]), pz, pbxy[r]);
         }
         if (BX == 64) {
             // buffer stores over this z plane: a lane whose point lies outside
@@ -453,18 +472,24 @@ __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__res
                                                       (int)((xin && yin[r]) ? foff[r] : kPastPlane),
                                                       0, kStoreNonTemporal);
             if (splane) {
+                // the rows' sign words in one store: lane r holds row r's
+                // word (one compare per row gives the wave's mask of lanes with
+                // g < iso; 4 is the ordered less-than predicate, NaN compares
+                // false), every other lane an offset past the plane
                 const __amdgpu_buffer_rsrc_t sr =
                     __builtin_amdgcn_make_buffer_rsrc(splane + (uint64_t)i * sstep, (short)0,
                                                       (int)(uint32_t)sstep, kBufferWord3);
+                uint64_t mine = 0;
+                uint32_t moff = kPastPlane;
 #pragma unroll
                 for (int r = 0; r < RY; ++r) {
                     const uint64_t m = __builtin_amdgcn_fcmpf(gv[r], c.iso, 4) & in_mask;
-                    const U32x2 w = {(uint32_t)m, (uint32_t)(m >> 32)};
-                    __builtin_amdgcn_raw_buffer_store_b64(
-                        w, sr, (int)((sign_lane && yin[r]) ? soff[r] + 8u * (uint32_t)(x0 >> 6)
-                                                           : kPastPlane),
-                        0, 0);
+                    const bool me = signs != nullptr && lane == r && yin[r];
+                    mine = me ? m : mine;
+                    moff = me ? soff[r] + 8u * (uint32_t)(x0 >> 6) : moff;
                 }
+                const U32x2 w = {(uint32_t)mine, (uint32_t)(mine >> 32)};
+                __builtin_amdgcn_raw_buffer_store_b64(w, sr, (int)moff, 0, 0);
             }
             continue;
         }

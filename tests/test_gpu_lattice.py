@@ -38,6 +38,18 @@ def _case_list():
                                         t0=0.25, modes=CF.fourier_modes(7))))
     cases.append(("rot_diamond", CF.make(65, 64, 31, 2, 14.0, 20.0, rot0=-0.5, rot1=0.5,
                                          sheet=False, prim=2, op=3, half=(0.3, 0.35, 0.4))))
+    # squared primitive distances of 0 and below 2^-96, where the field kernel
+    # takes its rescaled square root: a sphere centred on grid point
+    # (32, 32, 16), a box whose inside is at distance 0, and a centre 1e-20
+    # from the origin (subnormal squares at the first points)
+    cases.append(("sphere_on_point", CF.make(65, 65, 33, 1, 12.0, 20.0, rot0=0.1, rot1=0.9,
+                                             sheet=True, t0=0.3, prim=1, op=1,
+                                             center=(0.5, 0.5, 0.5), radius=0.3)))
+    cases.append(("box_inside", CF.make(65, 65, 33, 2, 14.0, rot0=0.2, rot1=0.6, sheet=True,
+                                        t0=0.2, prim=2, op=3, center=(0.5, 0.5, 0.5),
+                                        half=(0.2, 0.2, 0.2))))
+    cases.append(("sphere_near_origin", CF.make(64, 40, 24, 0, 10.0, sheet=False, prim=1, op=2,
+                                                center=(1e-20, 1e-21, 0.0), radius=0.1)))
     return cases
 
 
