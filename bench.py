@@ -794,6 +794,12 @@ def main():
                      "valu_wave_instructions": None if exe is None else
                      exe["valu_wave_instructions"],
                      "launch_ms": round(field_ms, 4),
+                     # VALU utilisation: the launch's VALU wave-instructions over
+                     # the issue slots of 1024 SIMDs in the live launch time at
+                     # one wave64 instruction per 2 cycles and the 2.4 GHz peak
+                     # clock (a lower bound: the clock under load is lower)
+                     "valu_issue_frac": None if exe is None else round(
+                         exe["valu_wave_instructions"] / (1024 * field_ms * 1e-3 * 2.4e9 / 2), 4),
                      "accounted": {"flop_per_point": FLOP_PER_POINT["varying_schwarz_p"],
                                    "tflops": round(field_tflops, 2),
                                    "frac": round(field_tflops / PEAK_FP32_TFLOPS, 4)}},
