@@ -33,9 +33,9 @@ def grid_args(gc, nx, ny, nz):
 
 
 class DropinBuffers:
-    def __init__(self, n_cells, max_verts, dev):
+    def __init__(self, n_cells, max_verts, dev, scan_fill=0):
         self.verts = torch.zeros(n_cells, dtype=torch.int32, device=dev)
-        self.verts_scan = torch.zeros_like(self.verts)
+        self.verts_scan = torch.full_like(self.verts, scan_fill)
         self.occ = torch.zeros_like(self.verts)
         self.occ_scan = torch.zeros_like(self.verts)
         self.comp = torch.zeros_like(self.verts)
@@ -44,12 +44,13 @@ class DropinBuffers:
 
 
 def run_dropin(gc, mode, vol, nx, ny, nz, iso, vol1=None, vol2=None, lo1=0.0, hi1=0.0, lo2=0.0,
-               hi2=0.0, voxel_size=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), max_verts=None):
+               hi2=0.0, voxel_size=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), max_verts=None,
+               scan_fill=0):
     dev = vol.device
     n = (nx - 1) * (ny - 1) * (nz - 1)
     if max_verts is None:
         max_verts = max(nx * ny * nz * 4, 300000)
-    b = DropinBuffers(n, max_verts, dev)
+    b = DropinBuffers(n, max_verts, dev, scan_fill)
     gs, sh, mk = grid_args(gc, nx, ny, nz)
     active, total = C.c_uint32(0), C.c_uint32(0)
     vs, ce = gc.Float3(*voxel_size), gc.Float3(*center)
@@ -200,6 +201,53 @@ def test_zero_active_voxels_leaves_later_buffers_alone(gc, dev):
         0.3, 0.0, 0.0))
     torch.cuda.synchronize()
     assert active.value == 0 and total.value == 0
+
+
+def ball_field(nx, ny, nz, cz, r):
+    """Signed distance to a ball centred at (nx/2, ny/2, cz) of radius r."""
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    d = np.sqrt((x - 0.5 * nx) ** 2 + (y - 0.5 * ny) ** 2 + (z - cz) ** 2) - r
+    return np.ascontiguousarray(d.astype(np.float32).reshape(-1))
+
+
+@pytest.mark.parametrize("mode", ["standard", "latticeone"])
+def test_scans_with_leading_empty_tiles(gc, orc, dev, mode):
+    """Grids whose first scan tiles (4096 cells each) hold no active cell:
+    verts_scan starts with other contents and must end as the oracle's, its
+    leading zeros included; calls of several sizes in a row (the scans'
+    scratch sums are cleared per call), then a call with no active cell,
+    which leaves verts_scan alone."""
+    cases = [(96, 90, 80, 0.8, 9.0), (61, 70, 66, 0.35, 12.0), (130, 40, 150, 0.9, 6.5),
+             (33, 33, 33, 0.5, 6.0)]
+    for nx, ny, nz, cz, r in cases:
+        vol_np = ball_field(nx, ny, nz, cz * nz, r)
+        print("leading empty tiles", mode, nx, ny, nz, flush=True)
+        vol = torch.from_numpy(vol_np).to(dev)
+        if mode == "standard":
+            b, active, total = run_dropin(gc, mode, vol, nx, ny, nz, 0.0, scan_fill=0x5A5A5A5A)
+            ref = orc.compute_isosurface(orc.MODE_STD, vol_np, nx, ny, nz, 0.0,
+                                         max_verts=b.pos.shape[0])
+        else:
+            # a band of the normalised field around the sphere (|distance| < 1.5)
+            npts = nx * ny * nz
+            a_, b_ = min(0.0, float(vol_np.min())), max(0.0, float(vol_np.max()))
+            lo, hi = (-1.5 - a_) / (b_ - a_), (1.5 - a_) / (b_ - a_)
+            ind = torch.empty_like(vol)
+            nrm = torch.empty_like(vol)
+            gc.check(gc.lib.gpucad_normalise_four(gc.ptr(vol), gc.ptr(ind), gc.ptr(nrm), npts, nx,
+                                                  ny, nz, lo, hi))
+            r_ind, r_nrm = orc.normalise_four(vol_np, nx, ny, nz, lo, hi)
+            b, active, total = run_dropin(gc, mode, ind, nx, ny, nz, 0.25, vol1=nrm, lo1=lo,
+                                          hi1=hi, scan_fill=0x5A5A5A5A)
+            ref = orc.compute_isosurface(orc.MODE_BAND1, r_ind, nx, ny, nz, 0.25, vol1=r_nrm,
+                                         lo1=lo, hi1=hi, max_verts=b.pos.shape[0])
+        first = int(np.argmax(ref["occ"] != 0))
+        assert active > 0 and (nz < 40 or first >= 4 * 4096), first
+        compare_with_oracle(orc, b, active, total, ref)
+    vol = torch.ones(40 * 40 * 40, dtype=torch.float32, device=dev)
+    b, active, total = run_dropin(gc, "standard", vol, 40, 40, 40, 0.0, scan_fill=0x5A5A5A5A)
+    assert active == 0 and total == 0
+    assert torch.all(b.verts_scan == 0x5A5A5A5A)
 
 
 @pytest.mark.parametrize("n", [1, 2, 4095, 4096, 4097, 1_000_003, 9_000_000])

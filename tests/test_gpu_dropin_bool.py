@@ -115,6 +115,47 @@ def test_boolean_pipeline_matches_oracle(gc, orc, dev, name, flags, n):
     assert np.all(to_np(pos)[max(t, cleared + 1):] == 7.0)
 
 
+def test_boolean_scans_with_leading_empty_tiles(gc, orc, dev):
+    """make_region of a ball in the top of the grid (the first 137 scan tiles
+    hold no active cell); verts_scan starts with other contents and must end
+    as the oracle's, its leading zeros included."""
+    nx, ny, nz = 90, 84, 100
+    a = grid(nx, ny, nz, lambda x, y, z: np.sqrt((x - 0.5 * nx) ** 2 + (y - 0.5 * ny) ** 2 +
+                                                 (z - 0.85 * nz) ** 2) - 8.5)
+    npts = nx * ny * nz
+    g0 = np.zeros(npts, orc.GRID_POINT)
+    g0["val"] = np.where(a < ISO, -1, 1)
+    g0["t_y"] = np.where(np.arange(npts) % 5 == 0, 0.5, 0.0)
+    flags = 32
+    zeros = np.zeros(npts, np.float32)
+    ref = orc.bool_compute_isosurface(flags, g0, zeros, zeros, nx, ny, nz, ISO, LO, HI)
+    first = int(np.argmax(ref["occ"] != 0))
+    assert first >= 100 * 4096
+    ncell = (nx - 1) * (ny - 1) * (nz - 1)
+    max_verts = max(npts * 4, 300000)
+    i32 = lambda: torch.zeros(ncell, dtype=torch.int32, device="cuda")
+    verts, occ, occ_scan, comp = i32(), i32(), i32(), i32()
+    verts_scan = torch.full((ncell,), 0x5A5A5A5A, dtype=torch.int32, device="cuda")
+    pos = torch.zeros((max_verts, 4), dtype=torch.float32, device="cuda")
+    norm = torch.zeros_like(pos)
+    d_g = gp_tensor(orc, g0)
+    gs, sh, mk = grid_args(gc, nx, ny, nz)
+    active, total, nfacets = C.c_uint32(0), C.c_uint32(0), C.c_uint64(0)
+    gc.check(gc.lib.gpucad_compute_isosurface(
+        gc.ptr(pos), gc.ptr(norm), ISO, ncell, gc.ptr(verts), gc.ptr(verts_scan), gc.ptr(occ),
+        gc.ptr(occ_scan), gs, sh, mk, gc.Float3(1, 1, 1), gc.Float3(0, 0, 0), C.byref(active),
+        C.byref(total), gc.ptr(comp), max_verts, gc.ptr(d_g), None, None, LO, HI, flags,
+        C.byref(nfacets)))
+    torch.cuda.synchronize()
+    u = lambda t: to_np(t).view(np.uint32)
+    assert (active.value, total.value) == (ref["active"], ref["total"])
+    assert np.array_equal(u(verts_scan), ref["verts_scan"])
+    assert np.array_equal(u(occ_scan), ref["occ_scan"])
+    assert np.array_equal(u(comp)[:ref["active"]], ref["comp"][:ref["active"]])
+    t = ref["total"]
+    assert np.array_equal(to_np(pos)[:t].view(np.uint32), ref["pos"][:t].view(np.uint32))
+
+
 def test_stage_entry_points_boolean(gc, orc, dev):
     nx, ny, nz = 30, 27, 25
     a, b, lat = fields(nx, ny, nz)
