@@ -19,7 +19,8 @@ Prints one JSON line:
               time measured with HIP events on its stream, against the FP32
               vector peak; the flop/point contract's count (hoisted per-axis
               terms included) is reported beside it as "accounted", and the
-              extraction's HBM figure beside that
+              extraction's HBM figure beside that, with the rates a plain
+              device fill and copy reach in the same run
   cpu_baseline the oracle (host restatement of the reference arithmetic, OpenMP)
               timed on this host on a bounded z-slab of the same workload
 """
@@ -544,6 +545,36 @@ def bench_dropin(n=120, period=30.0, runs=30, cpu=True):
     return out
 
 
+def device_rates(mib=512, runs=20):
+    """What plain streaming kernels reach on this device: torch's fill
+    (writes only) and copy (reads and writes) of mib MiB, medians of runs
+    launches timed with events; bytes moved / time."""
+    n = mib << 18
+    src = torch.empty(n, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+
+    def median_ms(fn):
+        for _ in range(3):
+            fn()
+        ts = []
+        for _ in range(runs):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            b.synchronize()
+            ts.append(a.elapsed_time(b))
+        return statistics.median(ts)
+
+    fill_ms = median_ms(lambda: dst.fill_(2.0))
+    copy_ms = median_ms(lambda: dst.copy_(src))
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"mib": mib, "fill_write_GBs": round(4 * n / (fill_ms * 1e-3) / 1e9, 1),
+            "copy_GBs": round(8 * n / (copy_ms * 1e-3) / 1e9, 1)}
+
+
 def cpu_model():
     """The host CPU's model name (what lscpu prints as "Model name")."""
     try:
@@ -753,6 +784,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
     counts = pipe.counts.cpu().tolist()
+    rates = device_rates()
     npts = p.npoints()
     flops = FLOP_PER_POINT["varying_schwarz_p"] * npts
     field_tflops = flops / (field_ms * 1e-3) / 1e12
@@ -803,9 +835,13 @@ def main():
                      "accounted": {"flop_per_point": FLOP_PER_POINT["varying_schwarz_p"],
                                    "tflops": round(field_tflops, 2),
                                    "frac": round(field_tflops / PEAK_FP32_TFLOPS, 4)}},
+        # beside the HBM peak, what a plain copy (reads and writes) and fill
+        # (writes) reach on this device in the same run
         "extraction": {"ms": round(extract_ms, 4), "algorithmic_bytes": ext_bytes,
                        "achieved_GBs": round(ext_gbs, 1),
-                       "frac_hbm": round(ext_gbs / PEAK_HBM_GBS, 4)},
+                       "frac_hbm": round(ext_gbs / PEAK_HBM_GBS, 4),
+                       "device_rates": rates,
+                       "frac_of_copy": round(ext_gbs / rates["copy_GBs"], 4)},
         "mtri_per_s": round(counts[1] / (extract_ms * 1e-3) / 1e6, 1),
         "n_vertices": counts[0],
         "n_triangles": counts[1],
