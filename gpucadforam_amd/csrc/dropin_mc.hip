@@ -292,36 +292,77 @@ __device__ __forceinline__ uint32_t bit_window(const uint32_t *w, uint32_t p) {
     return __builtin_amdgcn_alignbit(w[k + 1], w[k], p & 31u);
 }
 
-// vertex counts of the group of cells c0 .. c0+3 (those at or beyond n
-// count 0) whose first cell is p; c0 < n.  Four 32-bit windows of the bits
-// from corner 0's point of the first cell, one per corner row, hold the
-// corners of all four cells unless the group crosses a plane (cells past
-// the end of the row are gridSize.x - gridSizeMask.x points further on;
-// rows of at least 4 cells, so a group crosses at most one row end).
-__device__ __forceinline__ void group_counts(uint32_t (&nv)[4], const uint32_t *bits, uint3 p,
-                                             uint64_t c0, uint32_t n, const CellWalk &cw,
-                                             const uint8_t *s_nv) {
+// The count passes index their LDS copy of numVertsTable by a cell's corner
+// bits in row order: bits 0-1 the (x, x+1) pair of row (y, z), 2-3 of row
+// (y+1, z), 4-5 of (y, z+1), 6-7 of (y+1, z+1), each pair as it lies in the
+// point bits.  The reference's cube index takes corners 3 and 2 (and 7 and
+// 6) the other way round, so the copy holds nverts[row_order_cube(i)] at i
+// (the swap is its own inverse).
+__device__ __forceinline__ uint32_t row_order_cube(uint32_t i) {
+    return (i & 0x33u) | ((i & 0x44u) << 1) | ((i & 0x88u) >> 1);
+}
+
+// corner bits in row order of the cell whose corner 0 is point lin
+__device__ __forceinline__ uint32_t row_order_bits(const uint32_t *bits, uint32_t lin,
+                                                   uint32_t gx, uint32_t gxy) {
+    return bit_pair(bits, lin) | bit_pair(bits, lin + gx) << 2 | bit_pair(bits, lin + gxy) << 4 |
+           bit_pair(bits, lin + gxy + gx) << 6;
+}
+
+// vertex counts of a thread's kGroups groups of 4 cells, group k starting at
+// cell g0 + k * kGroupStep (g0 < n; cells at or beyond n count 0), from the
+// row-order table s_nv: the four 32-bit bit windows of every group (one per
+// corner row, from corner 0's point of its first cell) are loaded first, so
+// all groups share one round trip, and each cell takes its 2-bit field of
+// each; a group that crosses a plane (or whose rows lie far apart) reads its
+// cells' corners one by one.  Cells past the end of a row are gridSize.x -
+// gridSizeMask.x points further on (rows of at least 4 cells, so a group
+// crosses at most one row end).
+__device__ __forceinline__ void tile_counts(uint32_t (&nv)[kGroups][4], const uint32_t *bits,
+                                            uint64_t g0, uint32_t n, const CellWalk &cw,
+                                            const uint8_t *s_nv) {
     const uint32_t skip = cw.gx - cw.mask.x;
-    if (p.x + 3 < cw.mask.x || (p.y + 1 < cw.mask.y && skip <= 27 && cw.mask.x >= 4)) {
-        const uint32_t lin = p.z * cw.gxy + p.y * cw.gx + p.x;
-        const uint32_t w0 = bit_window(bits, lin), w1 = bit_window(bits, lin + cw.gx);
-        const uint32_t w2 = bit_window(bits, lin + cw.gxy);
-        const uint32_t w3 = bit_window(bits, lin + cw.gxy + cw.gx);
+    const bool rows_near = skip <= 27 && cw.mask.x >= 4;
+    uint3 p = calc_grid_pos((uint32_t)g0, cw.shift, cw.mask);
+    uint3 pk[kGroups];
+    uint32_t w[kGroups][4];
+    bool fast[kGroups];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const uint32_t sh = c + (p.x + c >= cw.mask.x ? skip : 0u);
-            const uint32_t cube = ((w0 >> sh) & 3u) | swap2((w1 >> sh) & 3u) << 2 |
-                                  ((w2 >> sh) & 3u) << 4 | swap2((w3 >> sh) & 3u) << 6;
-            nv[c] = c0 + c < n ? s_nv[cube] : 0u;
-        }
-    } else {  // the group crosses a plane (or rows are far apart)
+    for (int k = 0; k < kGroups; ++k) {
+        const uint64_t c0 = g0 + (uint64_t)k * kGroupStep;
+        pk[k] = p;
+        fast[k] = p.x + 3 < cw.mask.x || (p.y + 1 < cw.mask.y && rows_near);
+        // (a group past the cells reads the first windows and counts 0)
+        const uint32_t lin = c0 < n ? p.z * cw.gxy + p.y * cw.gx + p.x : 0u;
+        w[k][0] = bit_window(bits, lin);
+        w[k][1] = bit_window(bits, lin + cw.gx);
+        w[k][2] = bit_window(bits, lin + cw.gxy);
+        w[k][3] = bit_window(bits, lin + cw.gxy + cw.gx);
+        walk(p, cw.dx, cw.dy, cw.dz, cw.mask);
+    }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            nv[c] = 0;
-            if (c0 + c < n) {
-                const uint32_t lin = p.z * cw.gxy + p.y * cw.gx + p.x;
-                nv[c] = s_nv[cube_from_bits(bits, lin, cw.gx, cw.gxy)];
-                walk(p, 1, 0, 0, cw.mask);
+    for (int k = 0; k < kGroups; ++k) {
+        const uint64_t c0 = g0 + (uint64_t)k * kGroupStep;
+        if (fast[k]) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t sh = c + (pk[k].x + c >= cw.mask.x ? skip : 0u);
+                const uint32_t idx = __builtin_amdgcn_ubfe(w[k][0], sh, 2) |
+                                     __builtin_amdgcn_ubfe(w[k][1], sh, 2) << 2 |
+                                     __builtin_amdgcn_ubfe(w[k][2], sh, 2) << 4 |
+                                     __builtin_amdgcn_ubfe(w[k][3], sh, 2) << 6;
+                nv[k][c] = c0 + c < n ? s_nv[idx] : 0u;
+            }
+        } else {
+            uint3 q = pk[k];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                nv[k][c] = 0;
+                if (c0 + c < n) {
+                    const uint32_t lin = q.z * cw.gxy + q.y * cw.gx + q.x;
+                    nv[k][c] = s_nv[row_order_bits(bits, lin, cw.gx, cw.gxy)];
+                    walk(q, 1, 0, 0, cw.mask);
+                }
             }
         }
     }
@@ -336,26 +377,20 @@ __global__ void __launch_bounds__(kScanThreads) tile_sums_bits_kernel(
     static_assert(kScanThreads == 256, "one table entry per thread");
     __shared__ uint8_t s_nv[256];
     __shared__ uint32_t part[2][kWaves];
-    s_nv[threadIdx.x] = nverts[threadIdx.x];
+    s_nv[threadIdx.x] = nverts[row_order_cube(threadIdx.x)];  // the row-order copy
     __syncthreads();
     const uint64_t g0 = (uint64_t)blockIdx.x * kScanTile + 4u * threadIdx.x;
     uint32_t sv = 0, so = 0;
     if (g0 < n) {
-        uint3 p = calc_grid_pos((uint32_t)g0, cw.shift, cw.mask);
+        uint32_t nv[kGroups][4];
+        tile_counts(nv, bits, g0, n, cw, s_nv);
 #pragma unroll
-        for (int k = 0; k < kGroups; ++k) {
-            const uint64_t c0 = g0 + (uint64_t)k * kGroupStep;
-            if (c0 < n) {
-                uint32_t nv[4];
-                group_counts(nv, bits, p, c0, n, cw, s_nv);
+        for (int k = 0; k < kGroups; ++k)
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    sv += nv[c];
-                    so += nv[c] > 0;
-                }
+            for (int c = 0; c < 4; ++c) {
+                sv += nv[k][c];
+                so += nv[k][c] > 0;
             }
-            walk(p, cw.dx, cw.dy, cw.dz, cw.mask);
-        }
     }
     sv = wave_sum(sv);
     so = wave_sum(so);
@@ -406,7 +441,7 @@ __global__ void __launch_bounds__(kScanThreads) classify_scan_bits_kernel(
     static_assert(kScanThreads == 256 && kChunks <= 64, "tile layout");
     __shared__ uint8_t s_nv[256];
     __shared__ uint32_t wtot[2][kChunks];
-    s_nv[threadIdx.x] = nverts[threadIdx.x];
+    s_nv[threadIdx.x] = nverts[row_order_cube(threadIdx.x)];  // the row-order copy
     __syncthreads();
     const int wave = threadIdx.x >> 6;
     const uint32_t lane = threadIdx.x & 63;
@@ -416,15 +451,7 @@ __global__ void __launch_bounds__(kScanThreads) classify_scan_bits_kernel(
     for (int k = 0; k < kGroups; ++k)
 #pragma unroll
         for (int c = 0; c < 4; ++c) nv[k][c] = 0;
-    if (g0 < n) {
-        uint3 p = calc_grid_pos((uint32_t)g0, cw.shift, cw.mask);
-#pragma unroll
-        for (int k = 0; k < kGroups; ++k) {
-            const uint64_t c0 = g0 + (uint64_t)k * kGroupStep;
-            if (c0 < n) group_counts(nv[k], bits, p, c0, n, cw, s_nv);
-            walk(p, cw.dx, cw.dy, cw.dz, cw.mask);
-        }
-    }
+    if (g0 < n) tile_counts(nv, bits, g0, n, cw, s_nv);
     // chunk (k, wave) holds 256 consecutive cells, 4 per lane
     uint32_t ex_o[kGroups], ex_v[kGroups];
 #pragma unroll
