@@ -372,16 +372,16 @@ struct Batch {
     }
 
     void release() {
-        if (exec) hipGraphExecDestroy(exec);
-        if (graph) hipGraphDestroy(graph);
-        if (cap) hipStreamDestroy(cap);
-        if (done) hipEventDestroy(done);
-        if (t0) hipEventDestroy(t0);
-        if (t1) hipEventDestroy(t1);
-        if (arena) hipFree(arena);
-        if (items_buf) hipFree(items_buf);
-        if (vertices) hipFree(vertices);
-        if (triangles) hipFree(triangles);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap) (void)hipStreamDestroy(cap);
+        if (done) (void)hipEventDestroy(done);
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        if (arena) (void)hipFree(arena);
+        if (items_buf) (void)hipFree(items_buf);
+        if (vertices) (void)hipFree(vertices);
+        if (triangles) (void)hipFree(triangles);
         exec = nullptr;
         graph = nullptr;
         cap = nullptr;
@@ -543,7 +543,7 @@ int batch_create(const gc_lattice_params *grids, int n, const gc_batch_options *
         hipGraph_t g = nullptr;
         const hipError_t ce = hipStreamEndCapture(b->cap, &g);
         if (st) {
-            if (g) hipGraphDestroy(g);
+            if (g) (void)hipGraphDestroy(g);
             return bail(st);
         }
         if (ce != hipSuccess || !g) return bail(fail(GPUCAD_E_HIP, "graph capture failed"));
@@ -650,8 +650,8 @@ int gpucad_batch_destroy(gc_batch *h) {
     gc::clear_error();
     if (!h) return GPUCAD_OK;
     if (h->impl) {
-        if (h->impl->done) hipEventSynchronize(h->impl->done);
-        if (h->impl->cap) hipStreamSynchronize(h->impl->cap);
+        if (h->impl->done) (void)hipEventSynchronize(h->impl->done);
+        if (h->impl->cap) (void)hipStreamSynchronize(h->impl->cap);
         h->impl->release();
         delete h->impl;
     }

@@ -364,8 +364,6 @@ __device__ __forceinline__ void lattice_brick(const FieldConsts &c, float *__res
     // lanes whose point lies in the grid (the sign mask keeps only these)
     const uint64_t in_mask = __ballot(xin);
 
-    constexpr int kZUnroll = RY == 1 ? 2 : 1;
-#pragma unroll kZUnroll
     for (int i = 0; i < zn; ++i) {
         const float *e = ztab + i * kZTab;
         // (x, z) work, shared by the lane's rows

@@ -55,7 +55,7 @@ public:
         return GPUCAD_OK;
     }
     ~Arena() {
-        if (base_) hipFree(base_);
+        if (base_) (void)hipFree(base_);
     }
     char *base() const { return base_; }
     uint64_t size() const { return size_; }
@@ -187,10 +187,10 @@ struct Server {
 
     ~Server() {
         for (auto &s : slots) {
-            if (s.stream) hipStreamSynchronize(s.stream);
-            if (s.ev) hipEventDestroy(s.ev);
-            if (s.stream) hipStreamDestroy(s.stream);
-            if (s.pinned) hipHostFree(s.pinned);
+            if (s.stream) (void)hipStreamSynchronize(s.stream);
+            if (s.ev) (void)hipEventDestroy(s.ev);
+            if (s.stream) (void)hipStreamDestroy(s.stream);
+            if (s.pinned) (void)hipHostFree(s.pinned);
         }
     }
 
@@ -693,11 +693,11 @@ int gpucad_plain_run_trace(const gc_lattice_params *reqs, const double *arrival_
     uint32_t *tris = nullptr;
     void *ws = nullptr;
     auto cleanup = [&]() {
-        if (field) hipFree(field);
-        if (signs) hipFree(signs);
-        if (verts) hipFree(verts);
-        if (tris) hipFree(tris);
-        if (ws) hipFree(ws);
+        if (field) (void)hipFree(field);
+        if (signs) (void)hipFree(signs);
+        if (verts) (void)hipFree(verts);
+        if (tris) (void)hipFree(tris);
+        if (ws) (void)hipFree(ws);
     };
     if (hipMalloc(&field, most * 4) != hipSuccess || hipMalloc(&signs, most_words * 8) != hipSuccess ||
         hipMalloc(&verts, vcap * 12) != hipSuccess || hipMalloc(&tris, tcap * 12) != hipSuccess ||
@@ -738,8 +738,8 @@ int gpucad_plain_run_trace(const gc_lattice_params *reqs, const double *arrival_
         st = gpucad_extract(&a, &c, stream);
         if (st == GPUCAD_E_OVERFLOW && c.n_vertices < 0xffffffffull) {
             // grow once and redo this request
-            hipFree(verts);
-            hipFree(tris);
+            (void)hipFree(verts);
+            (void)hipFree(tris);
             verts = nullptr;
             tris = nullptr;
             vcap = std::max(vcap, c.n_vertices);
@@ -759,8 +759,8 @@ int gpucad_plain_run_trace(const gc_lattice_params *reqs, const double *arrival_
         latency_s[i] = (gc::now_s() - start) - arrival_s[i];
     }
     *wall_s = gc::now_s() - start;
-    hipStreamSynchronize(stream);
-    hipStreamDestroy(stream);
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
     cleanup();
     return st;
 }
