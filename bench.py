@@ -13,14 +13,20 @@ milliseconds of load); the timed steps then measure the steady state.
 
 Prints one JSON line:
   value       Gvoxels/s = grid points / (field + extraction) time per step
-  roofline    the dominant kernel (field kernel, VALU-bound): the FP32 flops it
-              executes per launch (rocprofv3 VALU instruction counters,
-              profiles/valu_counters_per_dispatch.txt) / its average launch
-              time measured with HIP events on its stream, against the FP32
-              vector peak; the flop/point contract's count (hoisted per-axis
-              terms included) is reported beside it as "accounted", and the
-              extraction's HBM figure beside that, with the rates a plain
-              device fill and copy reach in the same run
+  roofline    the step's longest launch, measured live with HIP events on its
+              stream (both long kernels are under "kernels"):
+              - the emit launch of the extraction (HBM-bound): the
+                extraction's algorithmic bytes (4 B per point for the field
+                read once, 12 B per vertex and per triangle written) / its
+                time from an event the extraction records just before it to
+                the event after the pass, against the HBM peak;
+              - the field kernel (VALU-bound): the FP32 flops it executes per
+                launch (rocprofv3 VALU instruction counters,
+                profiles/valu_counters_per_dispatch.txt) / its launch time,
+                against the FP32 vector peak, with the flop/point contract's
+                count (hoisted per-axis terms included) as "accounted";
+              the whole extraction pass's HBM figure is reported beside them,
+              with the rates a plain device fill and copy reach in the same run
   cpu_baseline the oracle (host restatement of the reference arithmetic, OpenMP)
               timed on this host on a bounded z-slab of the same workload
 """
@@ -181,10 +187,12 @@ def settle(pipe, stream, chunk=10, min_s=0.5, max_s=3.0, tol=0.01):
 
 
 def timed_steps(pipe, stream, steps, rounds=3):
-    """(ms per step, field ms, extraction ms): first `steps` steps bracketed
-    by two events only, then `rounds` times as many again with a fenceless
-    event before and after each pass; each pass's time is the median over the
-    rounds of its mean per step (one round can be disturbed on its own)."""
+    """(ms per step, field ms, extraction ms, emit ms): first `steps` steps
+    bracketed by two events only, then `rounds` times as many again with a
+    fenceless event before and after each pass and one that the extraction
+    records just before its emit launch (its last launch); each time is the
+    median over the rounds of its mean per step (one round can be disturbed
+    on its own)."""
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -196,19 +204,23 @@ def timed_steps(pipe, stream, steps, rounds=3):
     torch.cuda.synchronize()
     ms = start.elapsed_time(end) / steps
     h = pipe.h
-    field, extract = [], []
+    field, extract, emit = [], [], []
     for _ in range(rounds):
-        ev = [[HipEvent() for _ in range(3)] for _ in range(steps)]
+        ev = [[HipEvent() for _ in range(4)] for _ in range(steps)]
         for k in range(steps):
             ev[k][0].record(h)
             pipe.field_pass()
             ev[k][1].record(h)
+            pipe.args.emit_event = ev[k][3].e.value
             pipe.extract_pass()
             ev[k][2].record(h)
+        pipe.args.emit_event = None
         torch.cuda.synchronize()
         field.append(sum(e[0].ms_to(e[1]) for e in ev) / steps)
         extract.append(sum(e[1].ms_to(e[2]) for e in ev) / steps)
-    return ms, statistics.median(field), statistics.median(extract)
+        emit.append(sum(e[3].ms_to(e[2]) for e in ev) / steps)
+    return (ms, statistics.median(field), statistics.median(extract),
+            statistics.median(emit))
 
 
 class Pipeline:
@@ -295,7 +307,7 @@ def bench_config(p, name, steps, warmup):
     for _ in range(warmup):
         pipe.field_pass()
         pipe.extract_pass()
-    ms, field_ms, extract_ms = timed_steps(pipe, stream, steps)
+    ms, field_ms, extract_ms, emit_ms = timed_steps(pipe, stream, steps)
     counts = pipe.counts.cpu().tolist()
     npts = p.npoints()
     flops = FLOP_PER_POINT[name] * npts
@@ -776,7 +788,7 @@ def main():
         pipe.extract_pass()
     torch.cuda.synchronize()
     t_wall = time.perf_counter()
-    ms, field_ms, extract_ms = timed_steps(pipe, stream, args.steps)
+    ms, field_ms, extract_ms, emit_ms = timed_steps(pipe, stream, args.steps)
     wall = time.perf_counter() - t_wall
     if args.calib:
         src = torch.empty(128 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -792,6 +804,41 @@ def main():
     exe_tflops = None if exe is None else exe["flops"] / (field_ms * 1e-3) / 1e12
     ext_bytes = 4 * npts + 12 * counts[0] + 12 * counts[1]
     ext_gbs = ext_bytes / (extract_ms * 1e-3) / 1e9
+    # the field kernel is bound by the FP32 vector ALUs (no matrix work on
+    # this path): its roofline is the FP32 vector peak, against the flops the
+    # kernel executes (counters)
+    field_rl = {"bound": "valu",
+                "achieved": None if exe is None else round(exe_tflops, 2),
+                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": None if exe is None else round(exe_tflops / PEAK_FP32_TFLOPS, 4),
+                "traffic": measured_traffic("lattice_field_kernel", "varying_schwarz_p_%d" % p.nx),
+                "kernel": "lattice_field_kernel",
+                "flops_per_launch": None if exe is None else exe["flops"],
+                "flops_source": "rocprofv3 SQ_INSTS_VALU_{FMA,ADD,MUL,TRANS}_F32 x 64 "
+                                "(profiles/valu_counters_per_dispatch.txt)",
+                "valu_wave_instructions": None if exe is None else exe["valu_wave_instructions"],
+                "launch_ms": round(field_ms, 4),
+                # VALU utilisation: the launch's VALU wave-instructions over the
+                # issue slots of 1024 SIMDs in the live launch time at one wave64
+                # instruction per 2 cycles and the 2.4 GHz peak clock (a lower
+                # bound: the clock under load is lower)
+                "valu_issue_frac": None if exe is None else round(
+                    exe["valu_wave_instructions"] / (1024 * field_ms * 1e-3 * 2.4e9 / 2), 4),
+                "accounted": {"flop_per_point": FLOP_PER_POINT["varying_schwarz_p"],
+                              "tflops": round(field_tflops, 2),
+                              "frac": round(field_tflops / PEAK_FP32_TFLOPS, 4)}}
+    # the emit launch (vertex and triangle stages) is bound by HBM: one launch
+    # processes the grid's points, vertices and triangles, credited at the
+    # extraction's per-unit bytes (the field read once, 4 B per point; 12 B
+    # per vertex and per triangle written); its time runs from an event the
+    # extraction records before the launch to the event after the pass
+    emit_gbs = ext_bytes / (emit_ms * 1e-3) / 1e9
+    emit_rl = {"bound": "hbm", "achieved": round(emit_gbs, 1), "peak": PEAK_HBM_GBS,
+               "unit": "GB/s", "frac": round(emit_gbs / PEAK_HBM_GBS, 4),
+               "traffic": measured_traffic("emit_kernel", "varying_schwarz_p_%d" % p.nx),
+               "kernel": "emit_kernel", "algorithmic_bytes": ext_bytes,
+               "bytes_per_unit": {"point": 4, "vertex": 12, "triangle": 12},
+               "launch_ms": round(emit_ms, 4)}
     rec = {
         "metric": "Gvoxels/s field+marching-cubes at 512^3",
         "value": round(npts / (ms * 1e-3) / 1e9, 3),
@@ -810,31 +857,10 @@ def main():
                                "sheet, intersect sphere r=0.45; indexed extraction"
                                % (p.nx, p.ny, p.nz),
                    "grid": [p.nx, p.ny, p.nz]},
-        # the field kernel is the longest kernel of the step and is bound by
-        # the FP32 vector ALUs (no matrix work on this path): its roofline is the
-        # FP32 vector peak, against the flops the kernel executes (counters)
-        "roofline": {"bound": "valu",
-                     "achieved": None if exe is None else round(exe_tflops, 2),
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": None if exe is None else round(exe_tflops / PEAK_FP32_TFLOPS, 4),
-                     "traffic": measured_traffic("lattice_field_kernel",
-                                                 "varying_schwarz_p_%d" % p.nx),
-                     "kernel": "lattice_field_kernel",
-                     "flops_per_launch": None if exe is None else exe["flops"],
-                     "flops_source": "rocprofv3 SQ_INSTS_VALU_{FMA,ADD,MUL,TRANS}_F32 x 64 "
-                                     "(profiles/valu_counters_per_dispatch.txt)",
-                     "valu_wave_instructions": None if exe is None else
-                     exe["valu_wave_instructions"],
-                     "launch_ms": round(field_ms, 4),
-                     # VALU utilisation: the launch's VALU wave-instructions over
-                     # the issue slots of 1024 SIMDs in the live launch time at
-                     # one wave64 instruction per 2 cycles and the 2.4 GHz peak
-                     # clock (a lower bound: the clock under load is lower)
-                     "valu_issue_frac": None if exe is None else round(
-                         exe["valu_wave_instructions"] / (1024 * field_ms * 1e-3 * 2.4e9 / 2), 4),
-                     "accounted": {"flop_per_point": FLOP_PER_POINT["varying_schwarz_p"],
-                                   "tflops": round(field_tflops, 2),
-                                   "frac": round(field_tflops / PEAK_FP32_TFLOPS, 4)}},
+        # the step's longest launch (measured live): the roofline of the
+        # contract is that kernel's; both long kernels are under "kernels"
+        "roofline": field_rl if field_ms > emit_ms else emit_rl,
+        "kernels": {"lattice_field_kernel": field_rl, "emit_kernel": emit_rl},
         # beside the HBM peak, what a plain copy (reads and writes) and fill
         # (writes) reach on this device in the same run
         "extraction": {"ms": round(extract_ms, 4), "algorithmic_bytes": ext_bytes,

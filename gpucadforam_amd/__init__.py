@@ -119,6 +119,7 @@ class ExtractArgs(C.Structure):
         ("workspace", C.c_void_p),
         ("workspace_bytes", C.c_uint64),
         ("d_counts", C.c_void_p),
+        ("emit_event", C.c_void_p),
     ]
 
 

@@ -1001,6 +1001,7 @@ int enqueue_extract(const gc_extract_args *a, hipStream_t stream) {
     const TriArgs ta = {signs, base, twave, bsum, tabs->tri_packed, res, a->triangles,
                         a->triangles ? a->tri_capacity : 0};
     GC_ARG(L.nb <= 0xffffffffull / 3, "grid too large");
+    if (a->emit_event) GC_HIP(hipEventRecord((hipEvent_t)a->emit_event, stream));
     hipLaunchKernelGGL(emit_kernel<OneGrid>, dim3(3 * (unsigned)L.nb), dim3(kBlock), 0, stream,
                        one, va, ta);
     GC_CHECK_LAUNCH();

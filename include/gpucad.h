@@ -160,6 +160,9 @@ typedef struct gc_extract_args {
     void *workspace;           /* device, >= gpucad_extract_workspace_bytes() */
     uint64_t workspace_bytes;
     uint64_t *d_counts;        /* device uint64[2] = {vertices, triangles}, or NULL */
+    void *emit_event;          /* optional hipEvent_t, recorded on the stream just before
+                                  the emit launch (the last and longest of the call's
+                                  launches), so that launch can be timed; or NULL */
 } gc_extract_args;
 
 typedef struct gc_mesh_counts { uint64_t n_vertices, n_triangles; } gc_mesh_counts;

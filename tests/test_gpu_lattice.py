@@ -150,6 +150,55 @@ def test_extract_overflow_reports_counts(gc, orc, dev):
     assert np.array_equal(to_np(verts[:vcap]).view(np.uint32), v_ref[:vcap].view(np.uint32))
 
 
+def test_extract_emit_event(gc, orc, dev):
+    """gc_extract_args.emit_event: a HIP event recorded on the stream just
+    before the emit launch, between an event before the call and one after
+    it; the mesh is the one made without the event."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+    hip.hipEventSynchronize.argtypes = [C.c_void_p]
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    hip.hipEventDestroy.argtypes = [C.c_void_p]
+    ev = [C.c_void_p() for _ in range(3)]
+    for e in ev:
+        assert hip.hipEventCreate(C.byref(e)) == 0
+    n = 96
+    p = CF.plain_gyroid(n)
+    f, s = gpu_field(gc, p)
+    ref = gc.extract(f, n, n, n, signs=s)
+    wsb = gc.extract_workspace_bytes(n, n, n)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    verts = torch.empty((ref.n_vertices, 3), dtype=torch.float32, device=dev)
+    tris = torch.empty((ref.n_triangles, 3), dtype=torch.int32, device=dev)
+    a = gc.ExtractArgs()
+    a.field, a.signs = f.data_ptr(), s.data_ptr()
+    a.nx = a.ny = a.nz = n
+    a.iso = 0.0
+    a.origin = gc.Float3(0, 0, 0)
+    a.spacing = gc.Float3(1 / (n - 1), 1 / (n - 1), 1 / (n - 1))
+    a.vertices, a.vert_capacity = verts.data_ptr(), ref.n_vertices
+    a.triangles, a.tri_capacity = tris.data_ptr(), ref.n_triangles
+    a.normals = None
+    a.workspace, a.workspace_bytes = ws.data_ptr(), wsb
+    a.d_counts = None
+    a.emit_event = ev[1].value
+    h = gc.stream_handle()
+    assert hip.hipEventRecord(ev[0], h) == 0
+    gc.check(gc.lib.gpucad_extract(C.byref(a), None, h))
+    assert hip.hipEventRecord(ev[2], h) == 0
+    assert hip.hipEventSynchronize(ev[2]) == 0
+    before, emit = C.c_float(), C.c_float()
+    assert hip.hipEventElapsedTime(C.byref(before), ev[0], ev[1]) == 0
+    assert hip.hipEventElapsedTime(C.byref(emit), ev[1], ev[2]) == 0
+    assert before.value >= 0.0 and emit.value > 0.0
+    assert torch.equal(verts.view(torch.int32), ref.vertices.view(torch.int32))
+    assert torch.equal(tris, ref.triangles.to(torch.int32))
+    for e in ev:
+        hip.hipEventDestroy(e)
+
+
 def test_extract_rejects_bad_arguments(gc, dev):
     import ctypes as C
     a = gc.ExtractArgs()
