@@ -339,6 +339,9 @@ struct BGenArgs {
     const uint32_t *active_dev = nullptr;  // when set, the active voxel count is read here
     const uint32_t *vbase = nullptr;       // when set, scan[comp[i]] is read as vbase[i]
     const uint32_t *bits = nullptr;        // when set, cube indices come from the point bits
+    // when set (and some voxel is active): scan[0 .. *zfill) are written with
+    // 0 first (the leading verts_scan entries the prefix path leaves)
+    const uint32_t *zfill = nullptr;
 };
 
 __device__ __forceinline__ float tparam(const GridPoint &p, int ax) {
@@ -362,6 +365,11 @@ __global__ void __launch_bounds__(64) bool_generate_kernel(BGenArgs a) {
     __shared__ uint4 s_tri[64];  // each lane's triTable row
     const uint32_t lane = threadIdx.x;
     const uint32_t active = a.active_dev ? *a.active_dev : a.active;
+    if (a.zfill && active) {
+        const uint32_t z = *a.zfill;
+        for (uint32_t i = blockIdx.x * 64 + lane; i < z; i += gridDim.x * 64)
+            ((uint32_t *)a.scan)[i] = 0;  // (the caller's verts_scan)
+    }
     const uint32_t nbat = (uint32_t)(((uint64_t)active + 63) / 64);
     uint32_t bat = blockIdx.x, bat_end = nbat, bat_step = gridDim.x;
     if (a.active_dev) {
@@ -679,6 +687,7 @@ int gpucad_compute_isosurface(gc_float4 *pos, gc_float4 *norm, float iso, uint32
     a.active_dev = w.tot;
     a.vbase = w.vbase;
     a.bits = (const uint32_t *)w.bits;
+    if (dropin_prefix_path(w, false)) a.zfill = w.tot + 2;
     hipLaunchKernelGGL(bool_generate_kernel, dim3(dropin_generate_blocks(n)), dim3(64), 0, 0, a);
     GC_CHECK_LAUNCH();
     GC_HIP(hipStreamSynchronize(0));

@@ -432,19 +432,56 @@ __device__ __forceinline__ void store4(uint32_t *p, uint64_t c0, const uint32_t 
 // the scans' third pass from the point bits: every array the reference's
 // classify, scans and compaction write (verts_scan only when some voxel is
 // active), each tile's cells classified again from the bits (in L2) rather
-// than read back; vbase[j] = verts_scan of the j-th active cell
+// than read back; vbase[j] = verts_scan of the j-th active cell.  The tile's
+// offsets are bsum_o/bsum_v[tile], and tot[0] says whether any voxel is
+// active; or, PREFIX, bsum_o/bsum_v hold each tile's own sums and the tile
+// adds up those of the tiles before it (its loads issued with the bit
+// windows'), writes its verts_scan entries once its own prefix shows an
+// active voxel, and the first tile with an active voxel records in tot[2]
+// how many leading entries (all zero) it left to the generate launch; the
+// last tile writes the totals to tot[0..1] and host words 0-1.
+template <bool PREFIX>
 __global__ void __launch_bounds__(kScanThreads) classify_scan_bits_kernel(
     ScanOut o, const uint32_t *bits, CellWalk cw, uint32_t n, const uint8_t *nverts,
-    const uint32_t *bsum_o, const uint32_t *bsum_v, const uint32_t *tot) {
+    const uint32_t *bsum_o, const uint32_t *bsum_v, uint32_t *tot, uint32_t *hw) {
     constexpr int kWaves = kScanThreads / 64;
     constexpr int kChunks = kGroups * kWaves;
     static_assert(kScanThreads == 256 && kChunks <= 64, "tile layout");
     __shared__ uint8_t s_nv[256];
     __shared__ uint32_t wtot[2][kChunks];
+    __shared__ uint32_t s_pre[2][kWaves], s_any;
+    // the earlier tiles' sums, four per thread and step (loads in flight
+    // together with the table load and the bit windows below)
+    uint32_t po = 0, pv = 0;
+    if (PREFIX) {
+        for (uint32_t i = threadIdx.x; i < blockIdx.x; i += 4 * kScanThreads) {
+            uint32_t qo[4], qv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t j = i + u * kScanThreads;
+                const bool in = j < blockIdx.x;
+                qo[u] = in ? bsum_o[j] : 0u;
+                qv[u] = in ? bsum_v[j] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                po += qo[u];
+                pv += qv[u];
+            }
+        }
+    }
     s_nv[threadIdx.x] = nverts[row_order_cube(threadIdx.x)];  // the row-order copy
     __syncthreads();
     const int wave = threadIdx.x >> 6;
     const uint32_t lane = threadIdx.x & 63;
+    if (PREFIX) {
+        po = wave_sum(po);
+        pv = wave_sum(pv);
+        if (lane == 0) {
+            s_pre[0][wave] = po;
+            s_pre[1][wave] = pv;
+        }
+    }
     const uint64_t g0 = (uint64_t)blockIdx.x * kScanTile + 4u * threadIdx.x;
     uint32_t nv[kGroups][4];
 #pragma unroll
@@ -474,15 +511,39 @@ __global__ void __launch_bounds__(kScanThreads) classify_scan_bits_kernel(
     if (threadIdx.x < 64) {
         const uint32_t vo = lane < kChunks ? wtot[0][lane] : 0u;
         const uint32_t vv = lane < kChunks ? wtot[1][lane] : 0u;
-        const uint32_t eo = wave_inclusive_scan(vo) - vo + bsum_o[blockIdx.x];
-        const uint32_t ev = wave_inclusive_scan(vv) - vv + bsum_v[blockIdx.x];
+        const uint32_t io = wave_inclusive_scan(vo), iv = wave_inclusive_scan(vv);
+        uint32_t bo, bv;  // the tile's offsets
+        if (PREFIX) {
+            bo = 0;
+            bv = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                bo += s_pre[0][w];
+                bv += s_pre[1][w];
+            }
+            const uint32_t ao = __shfl(io, 63, 64), av = __shfl(iv, 63, 64);
+            if (lane == 0) {
+                s_any = bo + ao != 0;
+                if (bo == 0 && ao != 0) tot[2] = blockIdx.x * (uint32_t)kScanTile;
+                if (blockIdx.x == gridDim.x - 1) {
+                    tot[0] = bo + ao;
+                    tot[1] = bv + av;
+                    hw[0] = bo + ao;
+                    hw[1] = bv + av;
+                }
+            }
+        } else {
+            bo = bsum_o[blockIdx.x];
+            bv = bsum_v[blockIdx.x];
+            if (lane == 0) s_any = tot[0] != 0;
+        }
         if (lane < kChunks) {
-            wtot[0][lane] = eo;
-            wtot[1][lane] = ev;
+            wtot[0][lane] = io - vo + bo;
+            wtot[1][lane] = iv - vv + bv;
         }
     }
     __syncthreads();
-    const bool any_active = tot[0] != 0;
+    const bool any_active = s_any != 0;
     const bool vec = o.vec != 0;
 #pragma unroll
     for (int k = 0; k < kGroups; ++k) {
@@ -712,6 +773,9 @@ struct GenArgs {
     uint32_t active, max_verts;
     const uint32_t *active_dev = nullptr;  // when set, the active voxel count is read here
     const uint32_t *vbase = nullptr;       // when set, scan[comp[i]] is read as vbase[i]
+    // when set (and some voxel is active): scan[0 .. *zfill) are written with
+    // 0 first (the leading verts_scan entries the prefix path leaves)
+    const uint32_t *zfill = nullptr;
     const uint8_t *tri;  // 256*16
     const uint8_t *nverts;
 };
@@ -742,6 +806,11 @@ __global__ void __launch_bounds__(64) generate_kernel(GenArgs a) {
     __shared__ uint4 s_tri[64];  // each lane's triTable row
     const uint32_t lane = threadIdx.x;
     const uint32_t active = a.active_dev ? *a.active_dev : a.active;
+    if (a.zfill && active) {
+        const uint32_t z = *a.zfill;
+        for (uint32_t i = blockIdx.x * 64 + lane; i < z; i += gridDim.x * 64)
+            ((uint32_t *)a.scan)[i] = 0;  // (the caller's verts_scan)
+    }
     const uint32_t nbat = (uint32_t)(((uint64_t)active + 63) / 64);
     uint32_t bat = blockIdx.x, bat_end = nbat, bat_step = gridDim.x;
     if (a.active_dev) {
@@ -986,18 +1055,24 @@ int scans(const DropinScratch &w, const uint32_t *verts, uint32_t *occ_scan, uin
     return GPUCAD_OK;
 }
 
+// On the prefix path (dropin_prefix_path) without the pass that scans the
+// tile sums; otherwise with it and, where a later buffer is NULL, the active
+// count read (it must be 0) before the third pass.
 int classify_scans(const DropinScratch &w, uint32_t *verts, uint32_t *occ, uint32_t *occ_scan,
                    uint32_t *verts_scan, uint32_t *comp, gc_uint3 s, gc_uint3 sh, gc_uint3 m,
                    uint32_t n, const DeviceTables *t, uint32_t *hw, bool later_null,
                    hipStream_t st) {
     const CellWalk cw = cell_walk(s, sh, m);
     const uint32_t *bits = (const uint32_t *)w.bits;
+    const bool prefix = dropin_prefix_path(w, later_null);
     hipLaunchKernelGGL(tile_sums_bits_kernel, dim3(w.nb), dim3(kScanThreads), 0, st, bits, cw, n,
                        t->num_verts, w.bsum_v, w.bsum_o);
     GC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(scan_sums2_kernel, dim3(1), dim3(kSumsThreads), 0, st, w.bsum_o, w.bsum_v,
-                       w.nb, w.tot, hw);
-    GC_CHECK_LAUNCH();
+    if (!prefix) {
+        hipLaunchKernelGGL(scan_sums2_kernel, dim3(1), dim3(kSumsThreads), 0, st, w.bsum_o,
+                           w.bsum_v, w.nb, w.tot, hw);
+        GC_CHECK_LAUNCH();
+    }
     if (later_null) {
         GC_HIP(hipStreamSynchronize(st));
         GC_ARG(hw[0] == 0, "NULL buffer");
@@ -1011,9 +1086,14 @@ int classify_scans(const DropinScratch &w, uint32_t *verts, uint32_t *occ, uint3
     o.vbase = w.vbase;
     auto a16 = [](const void *q) { return ((uintptr_t)q & 15u) == 0; };
     o.vec = a16(verts) && a16(occ) && a16(occ_scan) && a16(verts_scan);
-    hipLaunchKernelGGL(classify_scan_bits_kernel, dim3(w.nb), dim3(kScanThreads), 0, st, o, bits,
-                       cw, n, t->num_verts, (const uint32_t *)w.bsum_o, (const uint32_t *)w.bsum_v,
-                       (const uint32_t *)w.tot);
+    if (prefix)
+        hipLaunchKernelGGL(classify_scan_bits_kernel<true>, dim3(w.nb), dim3(kScanThreads), 0, st,
+                           o, bits, cw, n, t->num_verts, (const uint32_t *)w.bsum_o,
+                           (const uint32_t *)w.bsum_v, w.tot, hw);
+    else
+        hipLaunchKernelGGL(classify_scan_bits_kernel<false>, dim3(w.nb), dim3(kScanThreads), 0,
+                           st, o, bits, cw, n, t->num_verts, (const uint32_t *)w.bsum_o,
+                           (const uint32_t *)w.bsum_v, w.tot, hw);
     GC_CHECK_LAUNCH();
     return GPUCAD_OK;
 }
@@ -1093,6 +1173,7 @@ int compute_isosurface(const float *vol, gc_float4 *pos, gc_float4 *norm, float 
     a.max_verts = max_verts;
     a.tri = t->tri_table;
     a.nverts = t->num_verts;
+    if (!fixed && dropin_prefix_path(w, later_null)) a.zfill = w.tot + 2;
     if (!later_null) {  // (otherwise no voxel is active)
         hipLaunchKernelGGL(generate_kernel<MODE>, dim3(dropin_generate_blocks(n)), dim3(64), 0,
                            st, a);
@@ -1143,13 +1224,13 @@ int dropin_scratch(gc_uint3 s, uint32_t n, DropinScratch *out) {
     w.bits_blocks = (uint32_t)((np + kBitsBlockPoints - 1) / kBitsBlockPoints);
     const size_t bits_u64 = (size_t)w.bits_blocks * (kBitsBlockPoints / 64) + 1;
     void *ws = nullptr;
-    int r = scratch_arena(bits_u64 * 8 + (2 * (size_t)w.nb + 2 + (size_t)n) * 4, &ws);
+    int r = scratch_arena(bits_u64 * 8 + (2 * (size_t)w.nb + 4 + (size_t)n) * 4, &ws);
     if (r) return r;
     w.bits = (uint64_t *)ws;
     w.bsum_o = (uint32_t *)(w.bits + bits_u64);
     w.bsum_v = w.bsum_o + w.nb;
     w.tot = w.bsum_v + w.nb;
-    w.vbase = w.tot + 2;
+    w.vbase = w.tot + 4;
     *out = w;
     return GPUCAD_OK;
 }

@@ -72,14 +72,22 @@ int dropin_read_last_pair(const uint32_t *a, const uint32_t *b, uint32_t n, uint
 // vertex counts, compacts the active cells (with each one's vertex offset in
 // vbase) and leaves the totals (active voxels, vertices) in tot[0..1] and in
 // host words 0-1.  With no active cell, verts_scan, comp and vbase are not
-// written (and may be NULL).
+// written (and may be NULL).  With every later buffer present and at most
+// kDropinPrefixMaxTiles scan tiles (dropin_prefix_path), each tile of the
+// last pass sums the earlier tiles' sums itself, and the leading verts_scan
+// entries (zero) of tiles before the first active cell are left to the
+// generate launch: their count is tot[2] (see GenArgs::zfill).
 constexpr int kDropinBitsWaveWords = 16;  // 64-bit words of point bits per wave
+constexpr uint32_t kDropinPrefixMaxTiles = 4096;
 struct DropinScratch {
     uint64_t *bits;          // the point bits (written by bits_blocks x 256 threads,
                              // 64 x kDropinBitsWaveWords points per wave), then a padding word
-    uint32_t *bsum_o, *bsum_v, *tot, *vbase;
+    uint32_t *bsum_o, *bsum_v, *tot, *vbase;  // tot: active, vertices, zero-filled entries
     uint32_t bits_blocks, nb;
 };
+inline bool dropin_prefix_path(const DropinScratch &w, bool later_null) {
+    return !later_null && w.nb <= kDropinPrefixMaxTiles;
+}
 int dropin_scratch(gc_uint3 s, uint32_t n, DropinScratch *out);  // from the arena
 int dropin_classify_scans(const DropinScratch &w, uint32_t *verts, uint32_t *occ,
                           uint32_t *occ_scan, uint32_t *verts_scan, uint32_t *comp, gc_uint3 s,
