@@ -250,6 +250,21 @@ def test_scans_with_leading_empty_tiles(gc, orc, dev, mode):
     assert torch.all(b.verts_scan == 0x5A5A5A5A)
 
 
+def test_scans_above_the_prefix_limit(gc, orc, dev):
+    """A grid of more than 4096 scan tiles (299 x 239 x 239 cells, 4170
+    tiles), which scans its tile sums in one workgroup before the third pass
+    instead of each tile adding up the earlier tiles' sums; the first active
+    cell lies in tile 1473 and verts_scan starts with other contents."""
+    nx, ny, nz = 300, 240, 240
+    vol_np = ball_field(nx, ny, nz, 0.6 * nz, 60.0)
+    vol = torch.from_numpy(vol_np).to(dev)
+    b, active, total = run_dropin(gc, "standard", vol, nx, ny, nz, 0.0, max_verts=3_000_000,
+                                  scan_fill=0x5A5A5A5A)
+    ref = orc.compute_isosurface(orc.MODE_STD, vol_np, nx, ny, nz, 0.0, max_verts=3_000_000)
+    assert (nx - 1) * (ny - 1) * (nz - 1) > 4096 * 4096 and active > 0
+    compare_with_oracle(orc, b, active, total, ref)
+
+
 @pytest.mark.parametrize("n", [1, 2, 4095, 4096, 4097, 1_000_003, 9_000_000])
 def test_exclusive_scan_u32(gc, orc, dev, n):
     rng = np.random.default_rng(n)
